@@ -1,0 +1,526 @@
+// C ABI of the MI355X FFT library: aoclfftz_setup_* / execute / execute_io /
+// destroy / version (drop-in for the reference's api/aoclfftz_api.c) and the
+// GPU extensions of aoclfftz_gpu.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <sstream>
+#include <string>
+
+#include "../../include/aoclfftz.h"
+#include "../../include/aoclfftz_gpu.h"
+#include "plan.h"
+
+namespace {
+
+constexpr uint32_t kMagic = 0x46465a47u;  // "GZFF"
+
+struct Handle {
+    uint32_t magic = kMagic;
+    fftz::Plan* plan = nullptr;
+    std::mutex amu;                // serialises stream-mode calls
+    fftz::Slot* aslot = nullptr;   // slot used in stream mode
+    hipStream_t stream = nullptr;  // bound user stream (stream mode)
+    std::string info;
+};
+
+void report_error(const char* func, const char* fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    fprintf(stderr, "[ERROR] : aoclfftz : %s : ", func);
+    vfprintf(stderr, fmt, ap);
+    fprintf(stderr, "\n");
+    va_end(ap);
+}
+
+const char* status_name(int s)
+{
+    switch (s) {
+        case AOCLFFTZ_TIME_OUT: return "AOCLFFTZ_TIME_OUT";
+        case AOCLFFTZ_MEMORY_FAILURE: return "AOCLFFTZ_MEMORY_FAILURE";
+        case AOCLFFTZ_INVALID_INPUT: return "AOCLFFTZ_INVALID_INPUT";
+        case AOCLFFTZ_SETUP_FAILURE: return "AOCLFFTZ_SETUP_FAILURE";
+        case AOCLFFTZ_EXECUTION_FAILURE: return "AOCLFFTZ_EXECUTION_FAILURE";
+        case AOCLFFTZ_SUCCESS: return "AOCLFFTZ_SUCCESS";
+        default: return "Unrecognized error code";
+    }
+}
+
+// Descriptor validation with the reference's rules and order
+// (api/validate_problem.h:346-417).  Like the reference it normalises some
+// control fields in place (logger_mode, measure_stats, opt_level,
+// dynamic_load_model, num_threads) instead of rejecting them.
+template <typename Desc, typename Dim>
+int validate(Desc* pd)
+{
+    const char* F = "validate_problem";
+    if (pd == nullptr) return AOCLFFTZ_INVALID_INPUT;
+    if (pd->dim_rank <= 0) {
+        report_error(F, "Dim rank must be atleast 1");
+        return AOCLFFTZ_INVALID_INPUT;
+    }
+    if (pd->vec_rank <= 0) {
+        report_error(F, "Vec rank must be atleast 1");
+        return AOCLFFTZ_INVALID_INPUT;
+    }
+    if (pd->dims == nullptr) {
+        report_error(F, "Dims cannot be NULL");
+        return AOCLFFTZ_INVALID_INPUT;
+    }
+    for (int i = 0; i < pd->dim_rank; i++) {
+        const Dim& d = pd->dims[i];
+        if (d.n <= 0) {
+            report_error(F, "Dimension[%d]: size must be atleast 1", i);
+            return AOCLFFTZ_INVALID_INPUT;
+        }
+        if (d.in_stride <= 0) {
+            report_error(F, "Dimension[%d]: in_stride must be greater than zero", i);
+            return AOCLFFTZ_INVALID_INPUT;
+        }
+        if (d.out_stride <= 0) {
+            report_error(F, "Dimension[%d]: out_stride must be greater than zero", i);
+            return AOCLFFTZ_INVALID_INPUT;
+        }
+    }
+    if (pd->vecs == nullptr) {
+        report_error(F, "Vecs cannot be NULL");
+        return AOCLFFTZ_INVALID_INPUT;
+    }
+    for (int i = 0; i < pd->vec_rank; i++) {
+        const Dim& v = pd->vecs[i];
+        if (v.n < 0) {
+            report_error(F, "Vector[%d]: size cannot be negative", i);
+            return AOCLFFTZ_INVALID_INPUT;
+        }
+        if (v.in_stride <= 0) {
+            report_error(F, "Vector[%d]: in_stride must be greater than zero", i);
+            return AOCLFFTZ_INVALID_INPUT;
+        }
+        if (v.out_stride <= 0) {
+            report_error(F, "Vector[%d]: out_stride must be greater than zero", i);
+            return AOCLFFTZ_INVALID_INPUT;
+        }
+    }
+    aoclfftz_cntrl_params_t* c = &pd->cntrl_params;
+    if ((int)c->logger_mode < 0 || (int)c->logger_mode > 3) c->logger_mode = AOCLFFTZ_LOG_NONE;
+    if (c->measure_stats != 0) c->measure_stats = 0;
+    if (!c->opt_off && (c->opt_level < 0 || c->opt_level > 3)) c->opt_level = 3;
+    const aoclfftz_flags_t& fl = pd->flags;
+    if (fl.fft_type > 1) {
+        report_error(F, "fft_type can be Complex(0) or Real(1)");
+        return AOCLFFTZ_INVALID_INPUT;
+    }
+    if (fl.fft_direction > 1) {
+        report_error(F, "fft_direction can be Forward(0) or Backward(1)");
+        return AOCLFFTZ_INVALID_INPUT;
+    }
+    if (fl.storage_order) {
+        report_error(F, "Library does not support out-of-order outputs");
+        return AOCLFFTZ_INVALID_INPUT;
+    }
+    if (fl.fft_placement > 1) {
+        report_error(F, "fft_placement can be In-place(0) or Out-of-place(1)");
+        return AOCLFFTZ_INVALID_INPUT;
+    }
+    if (fl.transpose_mode) {
+        report_error(F, "Library does not support standalone transpose");
+        return AOCLFFTZ_INVALID_INPUT;
+    }
+    if (fl.bit_reproducibility > 1) {
+        report_error(F, "bit_reproducibility can be Disable(0) or Enable(1)");
+        return AOCLFFTZ_INVALID_INPUT;
+    }
+    if (!fl.fft_placement) {
+        if ((const void*)pd->in != (const void*)pd->out) {
+            report_error(F, "Input and output buffer must be the same for in-place problems");
+            return AOCLFFTZ_INVALID_INPUT;
+        }
+        if (!fl.fft_type) {
+            for (int i = 0; i < pd->dim_rank; i++)
+                if (pd->dims[i].in_stride != pd->dims[i].out_stride) {
+                    report_error(F, "For dimension[%d]: in_stride (%d) != out_stride (%d)", i,
+                                 (int)pd->dims[i].in_stride, (int)pd->dims[i].out_stride);
+                    return AOCLFFTZ_INVALID_INPUT;
+                }
+            for (int i = 0; i < pd->vec_rank; i++)
+                if (pd->vecs[i].in_stride != pd->vecs[i].out_stride) {
+                    report_error(F, "For vecs[%d]: in_stride (%d) != out_stride (%d)", i,
+                                 (int)pd->vecs[i].in_stride, (int)pd->vecs[i].out_stride);
+                    return AOCLFFTZ_INVALID_INPUT;
+                }
+        } else {
+            // R2C: real-side strides are twice the complex-side strides
+            // (dims[0] may also be equal); C2R the other way round.
+            const int64_t in_scale = fl.fft_direction == 0 ? 2 : 1;
+            const int64_t out_scale = fl.fft_direction == 1 ? 2 : 1;
+            for (int i = 0; i < pd->dim_rank; i++) {
+                const int64_t is = pd->dims[i].in_stride, os = pd->dims[i].out_stride;
+                if (is * out_scale != os * in_scale && !(i == 0 && is == os)) {
+                    report_error(F, "For dimension[%d]: in-place real strides inconsistent", i);
+                    return AOCLFFTZ_INVALID_INPUT;
+                }
+            }
+            for (int i = 0; i < pd->vec_rank; i++) {
+                const int64_t is = pd->vecs[i].in_stride, os = pd->vecs[i].out_stride;
+                if (is * out_scale != os * in_scale && !(i == 0 && pd->vecs[i].n == 1)) {
+                    report_error(F, "For vecs[%d]: in-place real strides inconsistent", i);
+                    return AOCLFFTZ_INVALID_INPUT;
+                }
+            }
+        }
+    } else if ((const void*)pd->in == (const void*)pd->out) {
+        report_error(F, "Input and output buffer cannot be the same for out-of-place problems");
+        return AOCLFFTZ_INVALID_INPUT;
+    }
+    if (pd->pthr_fft.dynamic_load_model != 0 && pd->pthr_fft.dynamic_load_model != 1)
+        pd->pthr_fft.dynamic_load_model = 1;
+    if (pd->pthr_fft.num_threads < 1) pd->pthr_fft.num_threads = 1;
+    return AOCLFFTZ_SUCCESS;
+}
+
+template <typename Desc>
+fftz::ProblemSpec spec_of(const Desc* pd, int prec_bytes)
+{
+    fftz::ProblemSpec spec;
+    spec.prec_bytes = prec_bytes;
+    spec.dir = pd->flags.fft_direction;
+    spec.inplace = pd->flags.fft_placement == 0;
+    if (pd->flags.fft_type == 0)
+        spec.type = fftz::FftType::C2C;
+    else
+        spec.type = spec.dir == 0 ? fftz::FftType::R2C : fftz::FftType::C2R;
+    for (int i = 0; i < pd->dim_rank; i++)
+        spec.dims.push_back({(int64_t)pd->dims[i].n, (int64_t)pd->dims[i].in_stride, (int64_t)pd->dims[i].out_stride});
+    for (int i = 0; i < pd->vec_rank; i++)
+        spec.vecs.push_back({(int64_t)pd->vecs[i].n, (int64_t)pd->vecs[i].in_stride, (int64_t)pd->vecs[i].out_stride});
+    return spec;
+}
+
+template <typename Desc, typename Dim>
+void* setup_common(Desc* pd, int prec_bytes, const char* fname)
+{
+    int rc = validate<Desc, Dim>(pd);
+    if (rc != AOCLFFTZ_SUCCESS) {
+        report_error(fname, "Problem descriptor validation failed with %s", status_name(rc));
+        return nullptr;
+    }
+    fftz::ProblemSpec spec = spec_of(pd, prec_bytes);
+    std::string err;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        (void)hipGetLastError();
+        report_error(fname, "no HIP device available");
+        return nullptr;
+    }
+    fftz::Plan* plan = fftz::plan_create(spec, err);
+    if (!plan) {
+        report_error(fname, "plan creation failed: %s", err.c_str());
+        return nullptr;
+    }
+    plan->setup_in = (void*)pd->in;
+    plan->setup_out = (void*)pd->out;
+    Handle* h = new Handle();
+    h->plan = plan;
+    return h;
+}
+
+Handle* as_handle(void* p)
+{
+    Handle* h = static_cast<Handle*>(p);
+    if (h == nullptr || h->magic != kMagic) return nullptr;
+    return h;
+}
+
+bool is_device_ptr(const void* p)
+{
+    hipPointerAttribute_t a;
+    hipError_t e = hipPointerGetAttributes(&a, p);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
+}
+
+aoclfftz_error_type run(Handle* h, void* in, void* out, const char* fname)
+{
+    fftz::Plan* p = h->plan;
+    if (p->noop) return AOCLFFTZ_SUCCESS;
+    if (p->spec.inplace && in != out) {
+        report_error(fname, "in-place plan executed with distinct buffers");
+        return AOCLFFTZ_EXECUTION_FAILURE;
+    }
+    const bool in_dev = is_device_ptr(in);
+    const bool out_dev = is_device_ptr(out);
+    hipError_t e;
+    if (in_dev && out_dev && h->stream) {
+        std::lock_guard<std::mutex> g(h->amu);
+        if (!h->aslot) {
+            h->aslot = fftz::plan_acquire_slot(p, &e);
+            if (!h->aslot) {
+                report_error(fname, "slot allocation failed: %s", hipGetErrorString(e));
+                return AOCLFFTZ_MEMORY_FAILURE;
+            }
+        }
+        e = fftz::plan_enqueue(p, h->aslot, in, out, h->stream);
+        if (e != hipSuccess) {
+            report_error(fname, "launch failed: %s", hipGetErrorString(e));
+            return AOCLFFTZ_EXECUTION_FAILURE;
+        }
+        return AOCLFFTZ_SUCCESS;
+    }
+    fftz::Slot* s = fftz::plan_acquire_slot(p, &e);
+    if (!s) {
+        report_error(fname, "slot allocation failed: %s", hipGetErrorString(e));
+        return AOCLFFTZ_MEMORY_FAILURE;
+    }
+    aoclfftz_error_type rc = AOCLFFTZ_SUCCESS;
+    void* din = in;
+    void* dout = out;
+    do {
+        if (!in_dev || !out_dev) {
+            // host buffers: stage through device memory (reported separately
+            // from the device-resident rate; PCIe-bound)
+            const size_t need_in = (size_t)p->in_extent_bytes;
+            const size_t need_out = (size_t)p->out_extent_bytes;
+            const size_t need = p->spec.inplace ? std::max(need_in, need_out) : need_in;
+            if (!in_dev) {
+                if (s->stage_in_bytes < need) {
+                    if (s->stage_in) (void)hipFree(s->stage_in);
+                    s->stage_in = nullptr;
+                    s->stage_in_bytes = 0;
+                    if (hipMalloc(&s->stage_in, need) != hipSuccess) {
+                        rc = AOCLFFTZ_MEMORY_FAILURE;
+                        break;
+                    }
+                    s->stage_in_bytes = need;
+                }
+                din = s->stage_in;
+                if (hipMemcpyAsync(din, in, need_in, hipMemcpyHostToDevice, s->stream) != hipSuccess) {
+                    rc = AOCLFFTZ_EXECUTION_FAILURE;
+                    break;
+                }
+            }
+            if (p->spec.inplace) {
+                dout = din;
+            } else if (!out_dev) {
+                if (s->stage_out_bytes < need_out) {
+                    if (s->stage_out) (void)hipFree(s->stage_out);
+                    s->stage_out = nullptr;
+                    s->stage_out_bytes = 0;
+                    if (hipMalloc(&s->stage_out, need_out) != hipSuccess) {
+                        rc = AOCLFFTZ_MEMORY_FAILURE;
+                        break;
+                    }
+                    s->stage_out_bytes = need_out;
+                }
+                dout = s->stage_out;
+                // strided outputs: keep the caller's untouched elements
+                if (hipMemcpyAsync(dout, out, need_out, hipMemcpyHostToDevice, s->stream) != hipSuccess) {
+                    rc = AOCLFFTZ_EXECUTION_FAILURE;
+                    break;
+                }
+            }
+        }
+        e = fftz::plan_enqueue(p, s, din, dout, s->stream);
+        if (e != hipSuccess) {
+            report_error(fname, "launch failed: %s", hipGetErrorString(e));
+            rc = AOCLFFTZ_EXECUTION_FAILURE;
+            break;
+        }
+        if (dout != out) {
+            if (hipMemcpyAsync(out, dout, (size_t)p->out_extent_bytes, hipMemcpyDeviceToHost, s->stream) !=
+                hipSuccess) {
+                rc = AOCLFFTZ_EXECUTION_FAILURE;
+                break;
+            }
+        }
+        e = hipStreamSynchronize(s->stream);
+        if (e != hipSuccess) {
+            report_error(fname, "execution failed: %s", hipGetErrorString(e));
+            rc = AOCLFFTZ_EXECUTION_FAILURE;
+        }
+    } while (0);
+    if (rc != AOCLFFTZ_SUCCESS) (void)hipStreamSynchronize(s->stream);
+    fftz::plan_release_slot(p, s);
+    return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+FFTZ_VOID* aoclfftz_setup_f(aoclfftz_prob_desc_f* problem)
+{
+    return setup_common<aoclfftz_prob_desc_f, aoclfftz_dim_t>(problem, 4, "aoclfftz_setup_f");
+}
+
+FFTZ_VOID* aoclfftz_setup_d(aoclfftz_prob_desc_d* problem)
+{
+    return setup_common<aoclfftz_prob_desc_d, aoclfftz_dim_t>(problem, 8, "aoclfftz_setup_d");
+}
+
+FFTZ_VOID* aoclfftz_setup_f_64_(aoclfftz_prob_desc_f_64_* problem)
+{
+    return setup_common<aoclfftz_prob_desc_f_64_, aoclfftz_dim_t_64_>(problem, 4, "aoclfftz_setup_f_64_");
+}
+
+FFTZ_VOID* aoclfftz_setup_d_64_(aoclfftz_prob_desc_d_64_* problem)
+{
+    return setup_common<aoclfftz_prob_desc_d_64_, aoclfftz_dim_t_64_>(problem, 8, "aoclfftz_setup_d_64_");
+}
+
+aoclfftz_error_type aoclfftz_execute(FFTZ_VOID* handle)
+{
+    Handle* h = as_handle(handle);
+    if (!h) return AOCLFFTZ_EXECUTION_FAILURE;
+    if (h->plan->setup_in == nullptr || h->plan->setup_out == nullptr) {
+        report_error("aoclfftz_execute", "One or both setup-time buffers are NULL; use aoclfftz_execute_io");
+        return AOCLFFTZ_EXECUTION_FAILURE;
+    }
+    return run(h, h->plan->setup_in, h->plan->setup_out, "aoclfftz_execute");
+}
+
+aoclfftz_error_type aoclfftz_execute_io(FFTZ_VOID* handle, FFTZ_VOID* in, FFTZ_VOID* out)
+{
+    Handle* h = as_handle(handle);
+    if (!h || in == nullptr || out == nullptr) return AOCLFFTZ_EXECUTION_FAILURE;
+    return run(h, in, out, "aoclfftz_execute_io");
+}
+
+FFTZ_VOID aoclfftz_destroy(FFTZ_VOID* handle)
+{
+    Handle* h = as_handle(handle);
+    if (!h) return;
+    h->magic = 0;
+    fftz::plan_destroy(h->plan);
+    delete h;
+}
+
+const FFTZ_CHAR* aoclfftz_version(FFTZ_VOID)
+{
+    return AOCLFFTZ_LIBRARY_VERSION " MI355X gfx950";
+}
+
+aoclfftz_error_type aoclfftz_gpu_set_stream(FFTZ_VOID* handle, FFTZ_VOID* stream)
+{
+    Handle* h = as_handle(handle);
+    if (!h) return AOCLFFTZ_EXECUTION_FAILURE;
+    std::lock_guard<std::mutex> g(h->amu);
+    h->stream = (hipStream_t)stream;
+    if (!stream && h->aslot) {
+        fftz::plan_release_slot(h->plan, h->aslot);
+        h->aslot = nullptr;
+    }
+    return AOCLFFTZ_SUCCESS;
+}
+
+aoclfftz_error_type aoclfftz_gpu_synchronize(FFTZ_VOID* handle)
+{
+    Handle* h = as_handle(handle);
+    if (!h) return AOCLFFTZ_EXECUTION_FAILURE;
+    hipError_t e = hipSuccess;
+    if (h->stream) e = hipStreamSynchronize(h->stream);
+    std::lock_guard<std::mutex> g(h->plan->mu);
+    for (fftz::Slot* s : h->plan->slots)
+        if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    return e == hipSuccess ? AOCLFFTZ_SUCCESS : AOCLFFTZ_EXECUTION_FAILURE;
+}
+
+aoclfftz_error_type aoclfftz_gpu_set_graph(FFTZ_VOID* handle, FFTZ_INT32 enable)
+{
+    Handle* h = as_handle(handle);
+    if (!h) return AOCLFFTZ_EXECUTION_FAILURE;
+    h->plan->use_graph = enable != 0;
+    return AOCLFFTZ_SUCCESS;
+}
+
+const FFTZ_CHAR* aoclfftz_gpu_plan_info(FFTZ_VOID* handle)
+{
+    Handle* h = as_handle(handle);
+    if (!h) return "";
+    std::ostringstream os;
+    fftz::Plan* p = h->plan;
+    os << (p->spec.prec_bytes == 4 ? "fp32" : "fp64") << " " << p->description << "passes=" << p->launches.size();
+    for (const auto& ln : p->launches) {
+        os << " [" << (ln.kind == fftz::KernelKind::Bluestein ? "bluestein" : "pass") << " L=" << ln.d.L << " r=";
+        for (int s = 0; s < ln.d.nst; s++) os << (s ? "x" : "") << ln.d.radix[s];
+        os << " C=" << ln.d.C << " TL=" << ln.d.TL << " grid=" << ln.grid << " block=" << ln.block
+           << " lds=" << ln.lds_bytes << ((ln.d.flags & fftz::kInCol) ? " incol" : "")
+           << ((ln.d.flags & fftz::kOutCol) ? " outcol" : "") << ((ln.d.flags & fftz::kTw4) ? " tw4" : "") << "]";
+    }
+    os << " work=" << p->work_bytes;
+    h->info = os.str();
+    return h->info.c_str();
+}
+
+FFTZ_INT64 aoclfftz_gpu_plan_json(FFTZ_VOID* problem, FFTZ_INT32 precision_bytes, FFTZ_INT32 ilp64,
+                                  FFTZ_CHAR* buf, FFTZ_INT64 buflen)
+{
+    if (!problem || (precision_bytes != 4 && precision_bytes != 8)) return -1;
+    fftz::ProblemSpec spec;
+    if (ilp64) {
+        auto* pd = static_cast<aoclfftz_prob_desc_d_64_*>(problem);
+        if (validate<aoclfftz_prob_desc_d_64_, aoclfftz_dim_t_64_>(pd) != AOCLFFTZ_SUCCESS) return -1;
+        spec = spec_of(pd, precision_bytes);
+    } else {
+        auto* pd = static_cast<aoclfftz_prob_desc_d*>(problem);
+        if (validate<aoclfftz_prob_desc_d, aoclfftz_dim_t>(pd) != AOCLFFTZ_SUCCESS) return -1;
+        spec = spec_of(pd, precision_bytes);
+    }
+    std::string err;
+    fftz::Plan* plan = fftz::plan_create(spec, err, true);
+    if (!plan) {
+        report_error("aoclfftz_gpu_plan_json", "plan creation failed: %s", err.c_str());
+        return -1;
+    }
+    std::string js = fftz::plan_json(plan);
+    fftz::plan_destroy(plan);
+    if (buf && buflen > 0) {
+        const size_t n = std::min((size_t)(buflen - 1), js.size());
+        memcpy(buf, js.data(), n);
+        buf[n] = 0;
+    }
+    return (FFTZ_INT64)js.size();
+}
+
+aoclfftz_error_type aoclfftz_gpu_profile(FFTZ_VOID* handle, FFTZ_INT32 enable)
+{
+    Handle* h = as_handle(handle);
+    if (!h) return AOCLFFTZ_EXECUTION_FAILURE;
+    h->plan->profile = enable != 0;
+    return AOCLFFTZ_SUCCESS;
+}
+
+FFTZ_INT32 aoclfftz_gpu_profile_read(FFTZ_VOID* handle, float* ms, FFTZ_INT32 cap)
+{
+    Handle* h = as_handle(handle);
+    if (!h) return -1;
+    return fftz::plan_profile_read(h->plan, ms, cap);
+}
+
+FFTZ_INT32 aoclfftz_gpu_plan_launches(FFTZ_VOID* handle)
+{
+    Handle* h = as_handle(handle);
+    return h ? (FFTZ_INT32)h->plan->launches.size() : -1;
+}
+
+FFTZ_DOUBLE aoclfftz_gpu_plan_flops(FFTZ_VOID* handle)
+{
+    Handle* h = as_handle(handle);
+    return h ? h->plan->flops : 0.0;
+}
+
+FFTZ_INT64 aoclfftz_gpu_plan_device_bytes(FFTZ_VOID* handle)
+{
+    Handle* h = as_handle(handle);
+    if (!h) return 0;
+    return (FFTZ_INT64)h->plan->work_bytes;
+}
+
+}  // extern "C"

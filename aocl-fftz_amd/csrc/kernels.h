@@ -1,0 +1,16 @@
+// Kernel symbols exported from kernels.hip to the host planner.
+#pragma once
+
+namespace fftz {
+
+enum class KernelKind { Pass, Bluestein };
+
+// Device function symbol for hipLaunchKernel / hipFuncSetAttribute.
+// prec_bytes: 4 (complex64) or 8 (complex128).  all_radix: the variant that
+// carries the 3/5/6/7/10/11/12/13 stages (the power-of-two variant carries
+// only 2/4/8/16 and therefore needs fewer VGPRs).  E: complex values per
+// thread per stage (8 or 16).  maxt: block-size bound of the variant (256 or
+// 1024); the 256 variants get a larger VGPR budget.
+const void* kernel_symbol(KernelKind kind, int prec_bytes, bool all_radix, int E, int maxt);
+
+}  // namespace fftz

@@ -1,0 +1,1041 @@
+// GPU planner and executor.
+//
+// The reference's fixed-mode selector (selector/selector.c:509-789) walks a
+// gate list -- pow2-iterative, four-step, batched CT-L1 direct, batched,
+// N-D, Bluestein, direct, CT -- and builds a tree of CPU solvers.  On MI355X
+// the same decisions collapse into a few shapes of LDS pass:
+//
+//   axis length n with all prime factors <= 13 and n <= single-pass limit
+//       -> one pass along that axis (Stockham stages in LDS)
+//   1-D n above the limit
+//       -> four-step over p = 2 or 3 factors: column passes with the inter-pass
+//          twiddle fused into their store (pow2_fourstep_solver_dft.c:700-763,
+//          fused_twiddle_transpose_c.c:20-110), then a row pass whose store is
+//          the transpose
+//   axis length with a prime factor > 13
+//       -> one fused Bluestein pass (bluestein_solver_dft.c:137-234)
+//   R2C / C2R with even innermost n
+//       -> real-packed pass with recombine / separate fused
+//          (packed_solver_rdft.c:154-186)
+//   N-D -> one pass per axis (ndim_solver_dft.c:96-121), batch = line axes.
+#include "plan.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <map>
+#include <sstream>
+
+namespace fftz {
+
+// ---------------------------------------------------------------------------
+// Device info
+// ---------------------------------------------------------------------------
+const DeviceInfo& device_info()
+{
+    static DeviceInfo info;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess) return;
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return;
+        info.cus = prop.multiProcessorCount;
+        info.lds_per_block = prop.sharedMemPerBlock;
+        info.warp = prop.warpSize;
+        // Allow every kernel to use the full LDS of a workgroup.
+        for (int prec : {4, 8})
+            for (bool all : {false, true})
+                for (KernelKind k : {KernelKind::Pass, KernelKind::Bluestein})
+                    for (int e : {8, 16})
+                        for (int t : {256, 1024})
+                            (void)hipFuncSetAttribute(kernel_symbol(k, prec, all, e, t),
+                                                      hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                      (int)info.lds_per_block);
+    });
+    return info;
+}
+
+// ---------------------------------------------------------------------------
+// Roots of unity in double: exp(-2 pi i p / n), exact integer reduction and a
+// fold onto [0, pi/4] before sin/cos.
+// ---------------------------------------------------------------------------
+struct d2 {
+    double x, y;
+};
+
+static d2 root_fwd(int64_t p, int64_t n)
+{
+    p %= n;
+    if (p < 0) p += n;
+    const __int128 p8 = (__int128)p * 8;
+    const int64_t oct = (int64_t)(p8 / n);         // octant 0..7
+    const int64_t rem = (int64_t)(p8 - (__int128)oct * n);  // angle = (pi/4)(oct + rem/n)
+    double c, s;
+    const double a = (M_PI / 4.0) * ((double)rem / (double)n);
+    const double b = (M_PI / 4.0) * ((double)(n - rem) / (double)n);
+    // base angle theta = (pi/4) * (oct + rem/n); use the complementary
+    // angle on odd octants so the argument stays <= pi/4
+    double ct, st;  // cos, sin of theta
+    switch (oct) {
+        case 0: ct = cos(a); st = sin(a); break;
+        case 1: ct = sin(b); st = cos(b); break;
+        case 2: ct = -sin(a); st = cos(a); break;
+        case 3: ct = -cos(b); st = sin(b); break;
+        case 4: ct = -cos(a); st = -sin(a); break;
+        case 5: ct = -sin(b); st = -cos(b); break;
+        case 6: ct = sin(a); st = -cos(a); break;
+        default: ct = cos(b); st = -sin(b); break;
+    }
+    c = ct;
+    s = st;
+    return {c, -s};
+}
+
+// ---------------------------------------------------------------------------
+// Host FP64 FFT (recursive mixed radix) used only at setup, for the Bluestein
+// spectrum table.
+// ---------------------------------------------------------------------------
+static void host_fft_rec(int64_t n, const d2* in, int64_t is, d2* out)
+{
+    if (n == 1) { out[0] = in[0]; return; }
+    int64_t r = 0;
+    for (int64_t c : {4, 2, 3, 5, 7, 11, 13})
+        if (n % c == 0) { r = c; break; }
+    if (r == 0) {
+        for (int64_t k = 0; k < n; k++) {
+            d2 s = {0, 0};
+            for (int64_t j = 0; j < n; j++) {
+                d2 w = root_fwd(j * k, n);
+                d2 x = in[j * is];
+                s.x += x.x * w.x - x.y * w.y;
+                s.y += x.x * w.y + x.y * w.x;
+            }
+            out[k] = s;
+        }
+        return;
+    }
+    const int64_t m = n / r;
+    std::vector<d2> y((size_t)n);
+    for (int64_t q = 0; q < r; q++) host_fft_rec(m, in + q * is, is * r, y.data() + q * m);
+    for (int64_t k1 = 0; k1 < m; k1++)
+        for (int64_t k2 = 0; k2 < r; k2++) {
+            d2 s = {0, 0};
+            for (int64_t q = 0; q < r; q++) {
+                d2 w = root_fwd(q * (k1 + m * k2), n);
+                d2 x = y[(size_t)(q * m + k1)];
+                s.x += x.x * w.x - x.y * w.y;
+                s.y += x.x * w.y + x.y * w.x;
+            }
+            out[k1 + m * k2] = s;
+        }
+}
+
+// ---------------------------------------------------------------------------
+// Factorisation helpers
+// ---------------------------------------------------------------------------
+static const int kRadices[] = {16, 13, 12, 11, 10, 8, 7, 6, 5, 4, 3, 2};
+
+static bool smooth13(int64_t n)
+{
+    for (int64_t p : {2, 3, 5, 7, 11, 13})
+        while (n % p == 0) n /= p;
+    return n == 1;
+}
+
+// Fewest radix stages from kRadices (largest first on ties).
+static bool factor_radices(int L, std::vector<int>& out)
+{
+    out.clear();
+    if (L == 1) return true;
+    std::map<int, std::pair<int, int>> memo;
+    std::function<int(int)> solve = [&](int n) -> int {
+        if (n == 1) return 0;
+        auto it = memo.find(n);
+        if (it != memo.end()) return it->second.first;
+        int best = 1 << 30, br = 0;
+        for (int r : kRadices) {
+            if (n % r) continue;
+            int c = solve(n / r);
+            if (c + 1 < best) { best = c + 1; br = r; }
+        }
+        memo[n] = {best, br};
+        return best;
+    };
+    if (solve(L) >= (1 << 30)) return false;
+    for (int n = L; n > 1;) {
+        int r = memo[n].second;
+        out.push_back(r);
+        n /= r;
+    }
+    std::sort(out.begin(), out.end(), std::greater<int>());
+    return (int)out.size() <= kMaxStages;
+}
+
+static int next_pow2(int64_t v)
+{
+    int p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+static int ilog2(int v)
+{
+    int l = 0;
+    while ((1 << l) < v) l++;
+    return l;
+}
+
+// ---------------------------------------------------------------------------
+// Tables
+// ---------------------------------------------------------------------------
+struct TableBuilder {
+    std::vector<unsigned char>& buf;
+    int64_t add(const void* data, size_t bytes)
+    {
+        size_t off = (buf.size() + 255) & ~(size_t)255;
+        buf.resize(off + bytes);
+        memcpy(buf.data() + off, data, bytes);
+        return (int64_t)off;
+    }
+    int64_t add_complex(const std::vector<d2>& v, int prec)
+    {
+        if (prec == 8) return add(v.data(), v.size() * sizeof(d2));
+        std::vector<float> f(v.size() * 2);
+        for (size_t i = 0; i < v.size(); i++) {
+            f[2 * i] = (float)v[i].x;
+            f[2 * i + 1] = (float)v[i].y;
+        }
+        return add(f.data(), f.size() * sizeof(float));
+    }
+};
+
+// Stage twiddles for radices r[0..]: per stage, for c < ls, i in [1, R):
+// w_{R ls}^{i c} -- the per-column layout of core/common/twiddle.c:46-66.
+static int64_t stage_twiddles(TableBuilder& tb, int prec, const std::vector<int>& radix, int* twoff, int* ls_out)
+{
+    std::vector<d2> tw;
+    int ls = 1;
+    for (size_t s = 0; s < radix.size(); s++) {
+        const int R = radix[s];
+        twoff[s] = (int)tw.size();
+        ls_out[s] = ls;
+        for (int c = 0; c < ls; c++)
+            for (int i = 1; i < R; i++) tw.push_back(root_fwd((int64_t)i * c, (int64_t)R * ls));
+        ls *= R;
+    }
+    if (tw.empty()) tw.push_back({1.0, 0.0});
+    return tb.add_complex(tw, prec);
+}
+
+// Two-level FP64 table for w_N^p, p < N: lo[q] = w_N^q (q < 2^b), hi[h] = w_N^(h 2^b).
+static int64_t fourstep_twiddles(TableBuilder& tb, int64_t N, int* lobits)
+{
+    int b = 0;
+    while (((int64_t)1 << (2 * b)) < N) b++;
+    const int64_t lo_n = (int64_t)1 << b;
+    const int64_t hi_n = (N + lo_n - 1) / lo_n;
+    std::vector<d2> t((size_t)(lo_n + hi_n));
+    for (int64_t q = 0; q < lo_n; q++) t[(size_t)q] = root_fwd(q, N);
+    for (int64_t h = 0; h < hi_n; h++) t[(size_t)(lo_n + h)] = root_fwd(h * lo_n, N);
+    *lobits = b;
+    return tb.add(t.data(), t.size() * sizeof(d2));
+}
+
+// ---------------------------------------------------------------------------
+// Pass construction
+// ---------------------------------------------------------------------------
+struct LineAxis {
+    int64_t n, is, os;
+};
+
+struct PassReq {
+    KernelKind kind = KernelKind::Pass;
+    int L = 1;                 // transform length (M for Bluestein)
+    int nvalid_in = -1, nvalid_out = -1;
+    int64_t ies = 1, oes = 1;
+    std::vector<LineAxis> lines;
+    uint32_t flags = 0;
+    bool want_in_col = false, want_out_col = false;
+    Role in_role = Role::In, out_role = Role::Out;
+    int64_t tw4_n = 0;
+    int64_t chirp_off = -1, spec_off = -1, rtw_off = -1;
+    int extra_lds_elems = 0;   // C2R needs L+1 slots
+};
+
+static int env_int(const char* name, int dflt)
+{
+    const char* v = getenv(name);
+    return v ? atoi(v) : dflt;
+}
+
+// Merge line axes that form one dense run for both input and output.
+static std::vector<LineAxis> merge_axes(std::vector<LineAxis> ax)
+{
+    ax.erase(std::remove_if(ax.begin(), ax.end(), [](const LineAxis& a) { return a.n == 1; }), ax.end());
+    bool changed = true;
+    while (changed && ax.size() > 1) {
+        changed = false;
+        for (size_t i = 0; i < ax.size() && !changed; i++)
+            for (size_t j = 0; j < ax.size() && !changed; j++) {
+                if (i == j) continue;
+                if (ax[j].is == ax[i].is * ax[i].n && ax[j].os == ax[i].os * ax[i].n) {
+                    ax[i].n *= ax[j].n;
+                    ax.erase(ax.begin() + (long)j);
+                    changed = true;
+                }
+            }
+    }
+    return ax;
+}
+
+static bool build_pass(Plan* plan, TableBuilder& tb, const PassReq& q, std::string& err)
+{
+    const int prec = plan->spec.prec_bytes;
+    const int esize = 2 * prec;
+    const DeviceInfo& dev = device_info();
+    Launch ln;
+    ln.kind = q.kind;
+    PassDesc& d = ln.d;
+    d.L = q.L;
+    d.nvalid_in = q.nvalid_in < 0 ? q.L : q.nvalid_in;
+    d.nvalid_out = q.nvalid_out < 0 ? q.L : q.nvalid_out;
+    d.ies = q.ies;
+    d.oes = q.oes;
+    d.flags = q.flags;
+    d.scale_f = 1.0f;
+    d.scale_d = 1.0;
+
+    std::vector<int> radix;
+    if (!factor_radices(q.L, radix)) {
+        err = "length " + std::to_string(q.L) + " has no radix plan";
+        return false;
+    }
+    d.nst = (int)radix.size();
+    ln.all_radix = false;
+    for (size_t s = 0; s < radix.size(); s++) {
+        d.radix[s] = radix[s];
+        if (radix[s] != 2 && radix[s] != 4 && radix[s] != 8 && radix[s] != 16) ln.all_radix = true;
+    }
+    ln.stw_off = stage_twiddles(tb, prec, radix, d.twoff, d.ls);
+    for (int s = 0; s < d.nst; s++) d.lsdiv[s] = make_fastdiv((uint32_t)d.ls[s]);
+
+    // elements per thread: 16 for power-of-two radices; 8 for the mixed
+    // radix variant (fewer live VGPRs) unless that needs > 512 threads/line
+    int E = 16;
+    if (ln.all_radix && next_pow2((q.L + 7) / 8) <= 512) E = 8;
+    if (env_int("FFTZ_E", 0) == 8 || env_int("FFTZ_E", 0) == 16) {
+        const int e = env_int("FFTZ_E", 0);
+        if (next_pow2((q.L + e - 1) / e) <= 1024) E = e;
+    }
+    ln.E = E;
+    // threads per line
+    int TL = next_pow2((q.L + E - 1) / E);
+    if (TL < 1) TL = 1;
+    d.TL = TL;
+    d.tl_shift = ilog2(TL);
+    int need = q.L + q.extra_lds_elems;
+    int LP = lds_pad(need - 1) + 1;
+    if ((LP & 1) == 0) LP++;
+    d.LP = LP;
+
+    // line axes: choose l0
+    std::vector<LineAxis> ax = merge_axes(q.lines);
+    if (ax.empty()) ax.push_back({1, 0, 0});
+    size_t i0 = 0;
+    bool in_col = false, out_col = false;
+    if (q.want_in_col || q.want_out_col) {
+        for (size_t i = 0; i < ax.size(); i++) {
+            if ((q.want_in_col && ax[i].is == 1) || (!q.want_in_col && q.want_out_col && ax[i].os == 1)) {
+                i0 = i;
+                break;
+            }
+        }
+    } else {
+        for (size_t i = 1; i < ax.size(); i++)
+            if (std::llabs(ax[i].is) < std::llabs(ax[i0].is)) i0 = i;
+    }
+    std::swap(ax[0], ax[i0]);
+    // remaining axes sorted by input stride so l1 is the inner one
+    std::sort(ax.begin() + 1, ax.end(), [](const LineAxis& a, const LineAxis& b) { return a.is < b.is; });
+    int64_t n0 = ax[0].n;
+    in_col = q.want_in_col && ax[0].is == 1 && q.ies != 1;
+    out_col = q.want_out_col && ax[0].os == 1 && q.oes != 1;
+
+    // lines per tile
+    const size_t lds_cap = dev.lds_per_block;
+    const size_t line_bytes = (size_t)LP * esize;
+    if (line_bytes > lds_cap) {
+        err = "pass length " + std::to_string(q.L) + " exceeds LDS";
+        return false;
+    }
+    int C = 1;
+    if (in_col || out_col) {
+        const int cmax = env_int("FFTZ_COL_C", prec == 8 ? 16 : 32);
+        const size_t budget = std::min(lds_cap, (size_t)env_int("FFTZ_COL_LDS", 96 * 1024));
+        while (C * 2 <= cmax && (size_t)(C * 2) * line_bytes <= budget && (n0 % (C * 2)) == 0 &&
+               C * 2 * TL <= 1024)
+            C *= 2;
+        if (n0 % C != 0 || C < 2) { in_col = false; out_col = false; C = 1; }
+    }
+    if (!(in_col || out_col)) {
+        const int tgt = env_int("FFTZ_ROW_T", 256);
+        const size_t budget = std::min(lds_cap, (size_t)env_int("FFTZ_ROW_LDS", 64 * 1024));
+        int64_t nl_total = 1;
+        for (auto& a : ax) nl_total *= a.n;
+        while (C * TL < tgt && (size_t)(C * 2) * line_bytes <= budget && C * 2 * TL <= 1024 && C < nl_total) C *= 2;
+    }
+    if (C * TL > 1024) {
+        err = "tile too large";
+        return false;
+    }
+    d.C = C;
+    d.c_shift = ilog2(C);
+    if (in_col) d.flags |= kInCol;
+    if (out_col) d.flags |= kOutCol;
+
+    // line levels: l0, l1, l2 (+ host-looped outer axes)
+    std::vector<LineAxis> host_axes;
+    while (ax.size() > 3) {
+        host_axes.push_back(ax.back());
+        ax.pop_back();
+    }
+    while (ax.size() < 3) ax.push_back({1, 0, 0});
+    int64_t nl = ax[0].n * ax[1].n * ax[2].n;
+    if (nl >= ((int64_t)1 << 31)) {
+        err = "too many lines in one pass";
+        return false;
+    }
+    d.nlines = nl;
+    d.ln0 = make_fastdiv((uint32_t)ax[0].n);
+    d.ln1 = make_fastdiv((uint32_t)ax[1].n);
+    for (int k = 0; k < 3; k++) {
+        d.ils[k] = ax[k].is;
+        d.ols[k] = ax[k].os;
+    }
+    if (q.flags & kTw4) {
+        d.tw4_n = q.tw4_n;
+        ln.tw4_off = fourstep_twiddles(tb, q.tw4_n, &d.tw4_lobits);
+    }
+    ln.chirp_off = q.chirp_off;
+    ln.spec_off = q.spec_off;
+    ln.rtw_off = q.rtw_off;
+    ln.in_role = q.in_role;
+    ln.out_role = q.out_role;
+    ln.block = (uint32_t)(C * TL);
+    ln.maxt = ln.block <= 256 ? 256 : 1024;
+    const int64_t tiles = (nl + C - 1) / C;
+    ln.grid = (uint32_t)tiles;
+    ln.lds_bytes = (uint32_t)((size_t)C * line_bytes);
+    if ((in_col || out_col) && tiles >= 64 && env_int("FFTZ_XCD_REMAP", 1)) d.flags |= kXcdRemap;
+
+    // expand host-looped axes
+    std::vector<int64_t> ioffs{0}, ooffs{0};
+    const int64_t in_scale = ((q.flags & kRealIn) ? prec : esize);
+    const int64_t out_scale = ((q.flags & kRealOut) ? prec : esize);
+    for (auto& h : host_axes) {
+        std::vector<int64_t> ni, no;
+        for (size_t k = 0; k < ioffs.size(); k++)
+            for (int64_t j = 0; j < h.n; j++) {
+                ni.push_back(ioffs[k] + j * h.is * in_scale);
+                no.push_back(ooffs[k] + j * h.os * out_scale);
+            }
+        ioffs.swap(ni);
+        ooffs.swap(no);
+    }
+    for (size_t k = 0; k < ioffs.size(); k++) {
+        Launch l2 = ln;
+        l2.in_off_bytes = ioffs[k];
+        l2.out_off_bytes = ooffs[k];
+        plan->launches.push_back(l2);
+    }
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// Bluestein tables: chirp conj(B_k) (k < n) and spectrum conj(FFT_M(b)) / M.
+// B_k = exp(+i pi (k^2 mod 2n) / n) (core/common/bluestein_utils.c:88-158).
+// ---------------------------------------------------------------------------
+static int64_t bluestein_length(int64_t n)
+{
+    for (int64_t m = 2 * n - 1;; m++)
+        if (smooth13(m)) return m;
+}
+
+static void bluestein_tables(TableBuilder& tb, int prec, int64_t n, int64_t M, int64_t* chirp_off, int64_t* spec_off)
+{
+    std::vector<d2> b((size_t)M, d2{0, 0});
+    std::vector<d2> cb((size_t)n);
+    for (int64_t k = 0; k < n; k++) {
+        const int64_t sq = (int64_t)(((__int128)k * k) % (2 * n));
+        d2 w = root_fwd(sq, 2 * n);  // exp(-i pi sq / n) = conj(B_k)
+        cb[(size_t)k] = w;
+        d2 bk = {w.x, -w.y};
+        b[(size_t)k] = bk;
+        if (k > 0) b[(size_t)(M - k)] = bk;
+    }
+    std::vector<d2> fb((size_t)M);
+    host_fft_rec(M, b.data(), 1, fb.data());
+    const double inv = 1.0 / (double)M;
+    for (auto& v : fb) v = {v.x * inv, -v.y * inv};
+    *chirp_off = tb.add_complex(cb, prec);
+    *spec_off = tb.add_complex(fb, prec);
+}
+
+// ---------------------------------------------------------------------------
+// Plan construction
+// ---------------------------------------------------------------------------
+static int64_t single_pass_limit(int prec)
+{
+    return prec == 4 ? env_int("FFTZ_LMAX_F", 16384) : env_int("FFTZ_LMAX_D", 8192);
+}
+
+static int64_t extent(const std::vector<AxisSpec>& dims, const std::vector<AxisSpec>& vecs, bool use_in,
+                      int64_t n0_override)
+{
+    int64_t e = 0;
+    for (size_t i = 0; i < dims.size(); i++) {
+        int64_t n = (i == 0 && n0_override > 0) ? n0_override : dims[i].n;
+        e += (n - 1) * std::llabs(use_in ? dims[i].is : dims[i].os);
+    }
+    for (auto& v : vecs) e += (v.n - 1) * std::llabs(use_in ? v.is : v.os);
+    return e + 1;
+}
+
+// One complete 1-D transform along axis `ax` of a tensor as a list of passes.
+// `first`: the pass reads the caller's input geometry; otherwise it reads and
+// writes the output buffer in place.
+static bool plan_axis_c2c(Plan* plan, TableBuilder& tb, const std::vector<AxisSpec>& dims,
+                          const std::vector<AxisSpec>& vecs, size_t axis, bool first, bool conj_in, bool conj_out,
+                          Role in_role, Role out_role, bool use_out_strides_for_in, std::string& err)
+{
+    const int prec = plan->spec.prec_bytes;
+    const int64_t n = dims[axis].n;
+    PassReq q;
+    q.in_role = in_role;
+    q.out_role = out_role;
+    q.ies = use_out_strides_for_in ? dims[axis].os : dims[axis].is;
+    q.oes = dims[axis].os;
+    for (size_t j = 0; j < dims.size(); j++)
+        if (j != axis)
+            q.lines.push_back({dims[j].n, use_out_strides_for_in ? dims[j].os : dims[j].is, dims[j].os});
+    for (auto& v : vecs) q.lines.push_back({v.n, use_out_strides_for_in ? v.os : v.is, v.os});
+    (void)first;
+    if (conj_in) q.flags |= kConjIn;
+    if (conj_out) q.flags |= kConjOut;
+    q.want_in_col = q.ies != 1;
+    q.want_out_col = q.oes != 1;
+    if (smooth13(n)) {
+        if (n > single_pass_limit(prec)) {
+            err = "axis length " + std::to_string(n) + " above the single-pass limit in a multi-axis problem";
+            return false;
+        }
+        q.L = (int)n;
+        return build_pass(plan, tb, q, err);
+    }
+    // Bluestein
+    const int64_t M = bluestein_length(n);
+    if (M > single_pass_limit(prec)) {
+        err = "Bluestein length " + std::to_string(M) + " for prime factor length " + std::to_string(n) +
+              " exceeds the fused single-pass limit";
+        return false;
+    }
+    q.kind = KernelKind::Bluestein;
+    q.L = (int)M;
+    q.nvalid_in = (int)n;
+    q.nvalid_out = (int)n;
+    bluestein_tables(tb, prec, n, M, &q.chirp_off, &q.spec_off);
+    q.want_in_col = q.want_out_col = false;
+    return build_pass(plan, tb, q, err);
+}
+
+// Balanced factorisation of a large 1-D length into p factors, each <= fmax
+// and a product of radices.
+static bool split_factors(int64_t N, int64_t fmax, std::vector<int64_t>& f)
+{
+    for (int p = 2; p <= 3; p++) {
+        // candidate first factors near N^(1/p)
+        const double target = std::pow((double)N, 1.0 / p);
+        std::vector<int64_t> divs;
+        for (int64_t d = 2; d <= fmax && d <= N; d++)
+            if (N % d == 0 && smooth13(d)) divs.push_back(d);
+        std::sort(divs.begin(), divs.end(),
+                  [&](int64_t a, int64_t b) { return std::fabs(std::log((double)a / target)) < std::fabs(std::log((double)b / target)); });
+        for (int64_t d0 : divs) {
+            int64_t rest = N / d0;
+            if (p == 2) {
+                if (rest <= fmax && smooth13(rest)) {
+                    f = {d0, rest};
+                    return true;
+                }
+            } else {
+                const double t2 = std::sqrt((double)rest);
+                std::vector<int64_t> d2s;
+                for (int64_t d = 2; d <= fmax && d <= rest; d++)
+                    if (rest % d == 0 && smooth13(d) && rest / d <= fmax && smooth13(rest / d)) d2s.push_back(d);
+                if (d2s.empty()) continue;
+                std::sort(d2s.begin(), d2s.end(),
+                          [&](int64_t a, int64_t b) { return std::fabs(std::log((double)a / t2)) < std::fabs(std::log((double)b / t2)); });
+                f = {d0, d2s[0], rest / d2s[0]};
+                return true;
+            }
+        }
+    }
+    return false;
+}
+
+// Four-step over factors f[0..p-1] (p = 2 or 3): see pow2_fourstep_solver_dft.c
+// for the p = 2 form.  Step 0 reads the caller's input; the last step writes
+// the caller's output with the transpose folded into its store.
+static bool plan_large_1d(Plan* plan, TableBuilder& tb, const AxisSpec& dim, const std::vector<AxisSpec>& vecs,
+                          bool conj_in, bool conj_out, std::string& err)
+{
+    const int prec = plan->spec.prec_bytes;
+    const int64_t N = dim.n;
+    const int64_t fmax = prec == 8 ? env_int("FFTZ_FMAX_D", 512) : env_int("FFTZ_FMAX_F", 1024);
+    std::vector<int64_t> f;
+    if (!split_factors(N, fmax, f)) {
+        err = "no four-step factorisation for length " + std::to_string(N);
+        return false;
+    }
+    std::vector<AxisSpec> bv = vecs;
+    if (bv.size() > 1) {
+        err = "large 1-D transform with more than one batch axis";
+        return false;
+    }
+    int64_t B = bv.empty() ? 1 : bv[0].n;
+    int64_t bis = bv.empty() ? 0 : bv[0].is, bos = bv.empty() ? 0 : bv[0].os;
+    const size_t p = f.size();
+    std::vector<int64_t> M(p);  // M[s] = N / (f0...fs)
+    int64_t acc = N;
+    for (size_t s = 0; s < p; s++) {
+        acc /= f[s];
+        M[s] = acc;
+    }
+    plan->work_bytes = std::max(plan->work_bytes, (size_t)(N * B * 2 * prec));
+    // step 0
+    {
+        PassReq q;
+        q.L = (int)f[0];
+        q.ies = M[0] * dim.is;
+        q.oes = M[0];
+        q.lines.push_back({M[0], dim.is, 1});
+        if (B > 1) q.lines.push_back({B, bis, N});
+        q.flags = kTw4 | (conj_in ? kConjIn : 0);
+        q.tw4_n = N;
+        q.want_in_col = (dim.is == 1);
+        q.want_out_col = true;
+        q.in_role = Role::In;
+        q.out_role = Role::Work;
+        if (!build_pass(plan, tb, q, err)) return false;
+    }
+    // middle steps (p == 3)
+    for (size_t s = 1; s + 1 < p; s++) {
+        PassReq q;
+        q.L = (int)f[s];
+        q.ies = M[s];
+        q.oes = M[s];
+        q.lines.push_back({M[s], 1, 1});
+        int64_t rows = N / M[s - 1];
+        q.lines.push_back({rows, M[s - 1], M[s - 1]});
+        if (B > 1) q.lines.push_back({B, N, N});
+        q.flags = kTw4;
+        q.tw4_n = M[s - 1];
+        q.want_in_col = q.want_out_col = true;
+        q.in_role = Role::Work;
+        q.out_role = Role::Work;
+        if (!build_pass(plan, tb, q, err)) return false;
+    }
+    // last step: rows of length f[p-1], transposed store
+    {
+        PassReq q;
+        q.L = (int)f[p - 1];
+        q.ies = 1;
+        int64_t outer = N / f[p - 1];
+        q.oes = dim.os * outer;
+        // row (k0, k1): input offset k0*M0 + k1*M1, output (k0 + f0 k1) * os
+        q.lines.push_back({f[0], M[0], dim.os});
+        if (p == 3) q.lines.push_back({f[1], M[1], dim.os * f[0]});
+        if (B > 1) q.lines.push_back({B, N, bos});
+        q.flags = (conj_out ? kConjOut : 0);
+        q.want_in_col = false;
+        q.want_out_col = (dim.os == 1);
+        q.in_role = Role::Work;
+        q.out_role = Role::Out;
+        if (!build_pass(plan, tb, q, err)) return false;
+    }
+    std::ostringstream os;
+    os << "four-step";
+    for (auto v : f) os << " " << v;
+    plan->description += os.str() + "; ";
+    return true;
+}
+
+static bool plan_c2c(Plan* plan, TableBuilder& tb, std::vector<AxisSpec> dims, const std::vector<AxisSpec>& vecs,
+                     std::string& err)
+{
+    const int prec = plan->spec.prec_bytes;
+    const bool bwd = plan->spec.dir == 1;
+    if (dims.size() == 1 && smooth13(dims[0].n) && dims[0].n > single_pass_limit(prec))
+        return plan_large_1d(plan, tb, dims[0], vecs, bwd, bwd, err);
+    for (size_t a = 0; a < dims.size(); a++) {
+        const bool first = (a == 0);
+        if (!plan_axis_c2c(plan, tb, dims, vecs, a, first, bwd, bwd, first ? Role::In : Role::Out, Role::Out,
+                           !first, err))
+            return false;
+    }
+    plan->description += "row-column " + std::to_string(dims.size()) + "-D; ";
+    return true;
+}
+
+static bool plan_real(Plan* plan, TableBuilder& tb, std::vector<AxisSpec> dims, const std::vector<AxisSpec>& vecs,
+                      std::string& err)
+{
+    const int prec = plan->spec.prec_bytes;
+    const bool r2c = plan->spec.type == FftType::R2C;
+    const int64_t n0 = dims[0].n;
+    if (n0 % 2 != 0) {
+        err = "real transforms need an even innermost length";
+        return false;
+    }
+    const int64_t L = n0 / 2;
+    if (!smooth13(L) || L > single_pass_limit(prec)) {
+        err = "real-packed length " + std::to_string(L) + " unsupported";
+        return false;
+    }
+    std::vector<d2> rtw((size_t)(L / 2 + 1));
+    for (int64_t k = 0; k <= L / 2; k++) rtw[(size_t)k] = root_fwd(k, n0);
+    const int64_t rtw_off = tb.add_complex(rtw, prec);
+    // complex geometry of the half-spectrum side
+    std::vector<AxisSpec> cd = dims;  // n of dim 0 becomes L+1 bins on the complex side
+    if (r2c) {
+        PassReq q;
+        q.L = (int)L;
+        q.ies = dims[0].is;  // real elements
+        q.oes = dims[0].os;
+        for (size_t j = 1; j < dims.size(); j++) q.lines.push_back({dims[j].n, dims[j].is, dims[j].os});
+        for (auto& v : vecs) q.lines.push_back({v.n, v.is, v.os});
+        q.flags = kRealIn | kR2CPost;
+        q.rtw_off = rtw_off;
+        q.in_role = Role::In;
+        q.out_role = Role::Out;
+        if (!build_pass(plan, tb, q, err)) return false;
+        // remaining axes: C2C forward in place on the output
+        cd[0].n = L + 1;
+        for (size_t a = 1; a < dims.size(); a++) {
+            std::vector<AxisSpec> od = cd;
+            if (!plan_axis_c2c(plan, tb, od, vecs, a, false, false, false, Role::Out, Role::Out, true, err))
+                return false;
+        }
+        plan->description += "real-packed R2C; ";
+        return true;
+    }
+    // C2R: outer axes backward into the work buffer, then separate + FFT.
+    Role src = Role::In;
+    std::vector<AxisSpec> wdims = dims;
+    std::vector<AxisSpec> wvecs = vecs;
+    if (dims.size() > 1) {
+        // dense work layout: [vec][dims reversed], dim0 has L+1 bins
+        int64_t stride = 1;
+        std::vector<AxisSpec> dd = dims;
+        dd[0].n = L + 1;
+        for (size_t j = 0; j < dd.size(); j++) {
+            wdims[j] = {dd[j].n, dd[j].is, stride};
+            stride *= dd[j].n;
+        }
+        int64_t per = stride;
+        for (size_t v = 0; v < wvecs.size(); v++) {
+            wvecs[v] = {vecs[v].n, vecs[v].is, stride};
+            stride *= vecs[v].n;
+        }
+        plan->work_bytes = std::max(plan->work_bytes, (size_t)(stride * 2 * prec));
+        (void)per;
+        // copy-transform axis 1 from In to Work, then the rest in place on Work
+        for (size_t a = 1; a < dims.size(); a++) {
+            const bool first = (a == 1);
+            std::vector<AxisSpec> od = wdims;
+            if (!plan_axis_c2c(plan, tb, od, wvecs, a, first, true, true, first ? Role::In : Role::Work,
+                               Role::Work, !first, err))
+                return false;
+        }
+        src = Role::Work;
+        for (size_t j = 0; j < wdims.size(); j++) wdims[j].is = wdims[j].os;
+        for (size_t v = 0; v < wvecs.size(); v++) wvecs[v].is = wvecs[v].os;
+    }
+    PassReq q;
+    q.L = (int)L;
+    q.nvalid_in = (int)(L + 1);
+    q.extra_lds_elems = 1;
+    q.ies = wdims[0].is;
+    q.oes = dims[0].os;  // real elements
+    for (size_t j = 1; j < dims.size(); j++) q.lines.push_back({dims[j].n, wdims[j].is, dims[j].os});
+    for (size_t v = 0; v < vecs.size(); v++) q.lines.push_back({vecs[v].n, wvecs[v].is, vecs[v].os});
+    q.flags = kC2RPre | kConjOut | kRealOut;
+    q.rtw_off = rtw_off;
+    q.in_role = src;
+    q.out_role = Role::Out;
+    if (!build_pass(plan, tb, q, err)) return false;
+    plan->description += "real-packed C2R; ";
+    return true;
+}
+
+Plan* plan_create(const ProblemSpec& spec_in, std::string& err, bool dry_run)
+{
+    Plan* plan = new Plan();
+    plan->spec = spec_in;
+    ProblemSpec& spec = plan->spec;
+    if (!dry_run) {
+        (void)hipGetDevice(&plan->device);
+        (void)device_info();
+    }
+
+    // Drop size-1 dims (SHRINK_DIM_RANK, selector/selector.h:194-214) and
+    // size-1 batch axes; a zero-length batch axis is a no-op plan
+    // (selector/selector.c:1375-1385).
+    std::vector<AxisSpec> dims, vecs;
+    for (size_t i = 0; i < spec.dims.size(); i++)
+        if (spec.dims[i].n != 1 || (i == 0 && spec.type != FftType::C2C)) dims.push_back(spec.dims[i]);
+    if (dims.empty()) dims.push_back({1, 1, 1});
+    for (auto& v : spec.vecs) {
+        if (v.n == 0) plan->noop = true;
+        if (v.n > 1) vecs.push_back(v);
+    }
+    // extents for host staging
+    if (spec.type == FftType::C2C) {
+        plan->in_extent_bytes = extent(spec.dims, spec.vecs, true, -1) * 2 * spec.prec_bytes;
+        plan->out_extent_bytes = extent(spec.dims, spec.vecs, false, -1) * 2 * spec.prec_bytes;
+    } else if (spec.type == FftType::R2C) {
+        plan->in_extent_bytes = extent(spec.dims, spec.vecs, true, -1) * spec.prec_bytes;
+        plan->out_extent_bytes = extent(spec.dims, spec.vecs, false, spec.dims[0].n / 2 + 1) * 2 * spec.prec_bytes;
+    } else {
+        plan->in_extent_bytes = extent(spec.dims, spec.vecs, true, spec.dims[0].n / 2 + 1) * 2 * spec.prec_bytes;
+        plan->out_extent_bytes = extent(spec.dims, spec.vecs, false, -1) * spec.prec_bytes;
+    }
+    if (plan->noop) return plan;
+
+    // algorithmic flop count (5 N log2 N for C2C, 2.5 N log2 N for real)
+    double ntot = 1.0, nb = 1.0;
+    for (auto& d : spec.dims) ntot *= (double)d.n;
+    for (auto& v : spec.vecs) nb *= (double)v.n;
+    plan->flops = (spec.type == FftType::C2C ? 5.0 : 2.5) * ntot * std::log2(std::max(ntot, 2.0)) * nb;
+
+    TableBuilder tb{plan->host_tables};
+    bool ok;
+    if (spec.type == FftType::C2C)
+        ok = plan_c2c(plan, tb, dims, vecs, err);
+    else
+        ok = plan_real(plan, tb, dims, vecs, err);
+    if (!ok) {
+        delete plan;
+        return nullptr;
+    }
+    if (dry_run) return plan;
+    // upload tables and resolve pointers
+    if (!plan->host_tables.empty()) {
+        if (hipMalloc(&plan->tables, plan->host_tables.size()) != hipSuccess ||
+            hipMemcpy(plan->tables, plan->host_tables.data(), plan->host_tables.size(), hipMemcpyHostToDevice) !=
+                hipSuccess) {
+            err = "device allocation for plan tables failed";
+            if (plan->tables) (void)hipFree(plan->tables);
+            delete plan;
+            return nullptr;
+        }
+    }
+    char* base = (char*)plan->tables;
+    for (auto& ln : plan->launches) {
+        if (ln.stw_off >= 0) ln.d.stw = base + ln.stw_off;
+        if (ln.tw4_off >= 0) ln.d.tw4 = base + ln.tw4_off;
+        if (ln.chirp_off >= 0) ln.d.chirp = base + ln.chirp_off;
+        if (ln.spec_off >= 0) ln.d.spec = base + ln.spec_off;
+        if (ln.rtw_off >= 0) ln.d.rtw = base + ln.rtw_off;
+    }
+    plan->host_tables.clear();
+    plan->host_tables.shrink_to_fit();
+    plan->use_graph = env_int("FFTZ_GRAPH", 0) != 0;
+    return plan;
+}
+
+std::string plan_json(const Plan* p)
+{
+    std::ostringstream os;
+    os << "{\"noop\":" << (p->noop ? 1 : 0) << ",\"work_bytes\":" << p->work_bytes
+       << ",\"in_extent_bytes\":" << p->in_extent_bytes << ",\"out_extent_bytes\":" << p->out_extent_bytes
+       << ",\"prec\":" << p->spec.prec_bytes << ",\"launches\":[";
+    bool first = true;
+    for (const Launch& ln : p->launches) {
+        const PassDesc& d = ln.d;
+        os << (first ? "" : ",") << "{\"kind\":\"" << (ln.kind == KernelKind::Bluestein ? "bluestein" : "pass")
+           << "\",\"L\":" << d.L << ",\"nvalid_in\":" << d.nvalid_in << ",\"nvalid_out\":" << d.nvalid_out
+           << ",\"ies\":" << d.ies << ",\"oes\":" << d.oes << ",\"ils\":[" << d.ils[0] << "," << d.ils[1]
+           << "," << d.ils[2] << "],\"ols\":[" << d.ols[0] << "," << d.ols[1] << "," << d.ols[2]
+           << "],\"nlines\":" << d.nlines << ",\"ln0\":" << d.ln0.d << ",\"ln1\":" << d.ln1.d
+           << ",\"flags\":" << d.flags << ",\"C\":" << d.C << ",\"TL\":" << d.TL << ",\"LP\":" << d.LP
+           << ",\"E\":" << ln.E << ",\"grid\":" << ln.grid << ",\"block\":" << ln.block
+           << ",\"lds\":" << ln.lds_bytes << ",\"in_role\":" << (int)ln.in_role
+           << ",\"out_role\":" << (int)ln.out_role << ",\"in_off\":" << ln.in_off_bytes
+           << ",\"out_off\":" << ln.out_off_bytes << ",\"tw4_n\":" << d.tw4_n << ",\"radix\":[";
+        for (int s = 0; s < d.nst; s++) os << (s ? "," : "") << d.radix[s];
+        os << "],\"ls\":[";
+        for (int s = 0; s < d.nst; s++) os << (s ? "," : "") << d.ls[s];
+        os << "]}";
+        first = false;
+    }
+    os << "]}";
+    return os.str();
+}
+
+static void slot_free(Slot* s)
+{
+    if (s->graph) (void)hipGraphExecDestroy(s->graph);
+    if (s->work) (void)hipFree(s->work);
+    if (s->stage_in) (void)hipFree(s->stage_in);
+    if (s->stage_out) (void)hipFree(s->stage_out);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+}
+
+void plan_destroy(Plan* p)
+{
+    if (!p) return;
+    for (Slot* s : p->slots) {
+        if (s->stream) (void)hipStreamSynchronize(s->stream);
+        slot_free(s);
+    }
+    if (p->tables) (void)hipFree(p->tables);
+    for (hipEvent_t e : p->prof_events) (void)hipEventDestroy(e);
+    for (hipEvent_t e : p->prof_pool) (void)hipEventDestroy(e);
+    delete p;
+}
+
+Slot* plan_acquire_slot(Plan* p, hipError_t* err)
+{
+    std::lock_guard<std::mutex> g(p->mu);
+    *err = hipSuccess;
+    for (Slot* s : p->slots)
+        if (!s->busy) {
+            s->busy = true;
+            return s;
+        }
+    Slot* s = new Slot();
+    hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    if (e == hipSuccess && p->work_bytes) {
+        e = hipMalloc(&s->work, p->work_bytes);
+        s->work_bytes = p->work_bytes;
+    }
+    if (e != hipSuccess) {
+        *err = e;
+        slot_free(s);
+        return nullptr;
+    }
+    s->busy = true;
+    p->slots.push_back(s);
+    return s;
+}
+
+void plan_release_slot(Plan* p, Slot* s)
+{
+    std::lock_guard<std::mutex> g(p->mu);
+    s->busy = false;
+}
+
+static hipError_t record_prof(Plan* p, hipStream_t stream)
+{
+    hipEvent_t ev;
+    {
+        std::lock_guard<std::mutex> g(p->mu);
+        if (p->prof_pool.empty()) {
+            hipError_t e = hipEventCreate(&ev);
+            if (e != hipSuccess) return e;
+        } else {
+            ev = p->prof_pool.back();
+            p->prof_pool.pop_back();
+        }
+        p->prof_events.push_back(ev);
+    }
+    return hipEventRecord(ev, stream);
+}
+
+static hipError_t launch_all(Plan* p, Slot* s, const void* in, void* out, hipStream_t stream)
+{
+    if (p->profile) {
+        hipError_t e = record_prof(p, stream);
+        if (e != hipSuccess) return e;
+    }
+    for (const Launch& ln : p->launches) {
+        PassDesc d = ln.d;
+        auto role_ptr = [&](Role r) -> char* {
+            switch (r) {
+                case Role::In: return (char*)in;
+                case Role::Out: return (char*)out;
+                default: return (char*)s->work;
+            }
+        };
+        d.in = role_ptr(ln.in_role) + ln.in_off_bytes;
+        d.out = role_ptr(ln.out_role) + ln.out_off_bytes;
+        void* args[] = {&d};
+        hipError_t e = hipLaunchKernel(kernel_symbol(ln.kind, p->spec.prec_bytes, ln.all_radix, ln.E, ln.maxt),
+                                       dim3(ln.grid),
+                                       dim3(ln.block), args, ln.lds_bytes, stream);
+        if (e != hipSuccess) return e;
+        if (p->profile) {
+            e = record_prof(p, stream);
+            if (e != hipSuccess) return e;
+        }
+    }
+    return hipGetLastError();
+}
+
+int plan_profile_read(Plan* p, float* ms, int cap)
+{
+    std::vector<hipEvent_t> evs;
+    {
+        std::lock_guard<std::mutex> g(p->mu);
+        evs.swap(p->prof_events);
+    }
+    const size_t per = p->launches.size() + 1;  // events per execute
+    int n = 0;
+    for (size_t base = 0; base + per <= evs.size(); base += per) {
+        for (size_t i = 0; i + 1 < per; i++) {
+            float t = 0.f;
+            if (hipEventSynchronize(evs[base + i + 1]) != hipSuccess ||
+                hipEventElapsedTime(&t, evs[base + i], evs[base + i + 1]) != hipSuccess)
+                t = -1.f;
+            if (ms && n < cap) ms[n] = t;
+            n++;
+        }
+    }
+    std::lock_guard<std::mutex> g(p->mu);
+    for (hipEvent_t e : evs) p->prof_pool.push_back(e);
+    return n;
+}
+
+hipError_t plan_enqueue(Plan* p, Slot* s, const void* in, void* out, hipStream_t stream)
+{
+    if (p->noop) return hipSuccess;
+    if (!p->use_graph || p->profile) return launch_all(p, s, in, out, stream);
+    // One hipGraph per (slot, in, out): captured once, replayed afterwards.
+    if (!s->graph || s->graph_in != in || s->graph_out != out) {
+        if (s->graph) {
+            (void)hipGraphExecDestroy(s->graph);
+            s->graph = nullptr;
+        }
+        hipGraph_t g;
+        hipError_t e = hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) return e;
+        e = launch_all(p, s, in, out, s->stream);
+        hipError_t e2 = hipStreamEndCapture(s->stream, &g);
+        if (e != hipSuccess) return e;
+        if (e2 != hipSuccess) return e2;
+        e = hipGraphInstantiate(&s->graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return e;
+        s->graph_in = in;
+        s->graph_out = out;
+    }
+    return hipGraphLaunch(s->graph, stream);
+}
+
+}  // namespace fftz

@@ -1,0 +1,122 @@
+// GPU plan: the MI355X replacement for the reference's selector + solver tree
+// (selector/selector.c:1387-1620 prepare_and_setup_dft; core/solvers/**).
+//
+// A plan is a flat list of pass launches (see pass.h) plus one device arena
+// holding every twiddle / chirp table, built once at setup.  Execution binds
+// the caller's buffers to the launches and enqueues them on a HIP stream.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "kernels.h"
+#include "pass.h"
+
+namespace fftz {
+
+enum class Role : int { In = 0, Out = 1, Work = 2 };
+
+struct AxisSpec {
+    int64_t n, is, os;  // length, input stride, output stride (elements)
+};
+
+enum class FftType : int { C2C = 0, R2C = 1, C2R = 2 };
+
+struct ProblemSpec {
+    int prec_bytes = 4;  // 4: float, 8: double
+    FftType type = FftType::C2C;
+    int dir = 0;         // 0 forward, 1 backward
+    bool inplace = false;
+    std::vector<AxisSpec> dims;  // dims[0] innermost
+    std::vector<AxisSpec> vecs;  // batch axes
+};
+
+struct Launch {
+    KernelKind kind = KernelKind::Pass;
+    bool all_radix = false;
+    int E = 16;        // complex values per thread per stage
+    int maxt = 256;    // block-size bound of the kernel variant
+    PassDesc d{};
+    Role in_role = Role::In, out_role = Role::Out;
+    int64_t in_off_bytes = 0, out_off_bytes = 0;
+    uint32_t grid = 0, block = 0, lds_bytes = 0;
+    // table offsets in the plan arena (-1: none), resolved after upload
+    int64_t stw_off = -1, tw4_off = -1, chirp_off = -1, spec_off = -1, rtw_off = -1;
+};
+
+// Per-call execution context: a stream, the work buffer and host staging
+// buffers.  One slot per concurrent execute_io caller, so calls on a shared
+// handle never share scratch (the GPU analogue of the reference's per-call
+// scratch slab, core/common/memory_manager.h:53-175).
+struct Slot {
+    hipStream_t stream = nullptr;
+    void* work = nullptr;
+    size_t work_bytes = 0;
+    void* stage_in = nullptr;
+    void* stage_out = nullptr;
+    size_t stage_in_bytes = 0, stage_out_bytes = 0;
+    bool busy = false;
+    // graph cache for the last (in, out) pair run on this slot
+    hipGraphExec_t graph = nullptr;
+    const void* graph_in = nullptr;
+    void* graph_out = nullptr;
+};
+
+struct Plan {
+    ProblemSpec spec;
+    int device = 0;
+    bool noop = false;
+    std::vector<Launch> launches;
+    size_t work_bytes = 0;
+    int64_t in_extent_bytes = 0, out_extent_bytes = 0;
+    std::vector<unsigned char> host_tables;
+    void* tables = nullptr;
+    void* setup_in = nullptr;
+    void* setup_out = nullptr;
+    std::mutex mu;
+    std::vector<Slot*> slots;
+    hipStream_t user_stream = nullptr;  // set by aoclfftz_gpu_set_stream
+    bool use_graph = false;
+    // per-launch timing: when enabled, an event is recorded before the first
+    // launch and after each launch of every execute (graphs are bypassed)
+    bool profile = false;
+    std::vector<hipEvent_t> prof_events;  // recorded, not yet read
+    std::vector<hipEvent_t> prof_pool;    // free events
+    std::string description;            // human-readable plan summary
+    double flops = 0.0;                 // 5 N log2 N per transform x batch (C2C)
+};
+
+// Build a plan.  Returns nullptr and fills err on an unsupported problem.
+// dry_run: build the launch list only (no device memory, no HIP calls), for
+// host-side inspection of the plan geometry.
+Plan* plan_create(const ProblemSpec& spec, std::string& err, bool dry_run = false);
+
+// JSON description of every launch of a plan (geometry, strides, flags).
+std::string plan_json(const Plan* p);
+void plan_destroy(Plan* p);
+
+// Enqueue the plan on the slot's stream (or the user stream) for device
+// buffers.  Returns hipSuccess or the first HIP error.
+hipError_t plan_enqueue(Plan* p, Slot* s, const void* in, void* out, hipStream_t stream);
+
+// Read back per-launch durations (ms) recorded since the last read, in
+// launch order (execute-major).  Returns the number of durations.
+int plan_profile_read(Plan* p, float* ms, int cap);
+
+// Acquire / release an execution slot (creating one if all are busy).
+Slot* plan_acquire_slot(Plan* p, hipError_t* err);
+void plan_release_slot(Plan* p, Slot* s);
+
+// Device properties cached at first use.
+struct DeviceInfo {
+    int cus = 256;                  // MI355X defaults, used when no device
+    size_t lds_per_block = 163840;  // is visible (dry-run planning)
+    int warp = 64;
+};
+const DeviceInfo& device_info();
+
+}  // namespace fftz

@@ -1,0 +1,251 @@
+"""Python binding of the MI355X FFT library's C ABI (ctypes).
+
+The library is ``aocl-fftz_amd/lib/libaocl_fftz.so`` (built by build.py).  The
+binding exposes the reference's descriptor structs (api/aoclfftz.h:86-311)
+and entry points unchanged, plus a small ``Plan`` helper that drives them with
+torch tensors (device buffers) or numpy arrays (host buffers, staged by the
+library).  torch is only used for device memory and streams.
+
+Loading fails loudly when the library has not been built: there is no CPU
+fallback anywhere in the product path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional, Sequence, Tuple
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libaocl_fftz.so")
+
+SUCCESS = 0
+EXECUTION_FAILURE = -1
+SETUP_FAILURE = -2
+INVALID_INPUT = -3
+MEMORY_FAILURE = -4
+TIME_OUT = -5
+
+
+class Flags(ctypes.Structure):
+    _fields_ = [("fft_type", ctypes.c_uint8), ("fft_direction", ctypes.c_uint8),
+                ("storage_order", ctypes.c_uint8), ("fft_placement", ctypes.c_uint8),
+                ("transpose_mode", ctypes.c_uint8), ("bit_reproducibility", ctypes.c_uint8)]
+
+
+class Dim(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int32), ("in_stride", ctypes.c_int32), ("out_stride", ctypes.c_int32)]
+
+
+class Dim64(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_ssize_t), ("in_stride", ctypes.c_ssize_t), ("out_stride", ctypes.c_ssize_t)]
+
+
+class Smp(ctypes.Structure):
+    _fields_ = [("num_threads", ctypes.c_int32), ("dynamic_load_model", ctypes.c_uint32)]
+
+
+class Cntrl(ctypes.Structure):
+    _fields_ = [("opt_level", ctypes.c_int32), ("opt_off", ctypes.c_int32),
+                ("logger_mode", ctypes.c_int32), ("measure_stats", ctypes.c_int32)]
+
+
+def _desc(dim_t):
+    class ProbDesc(ctypes.Structure):
+        _fields_ = [("in_", ctypes.c_void_p), ("out", ctypes.c_void_p),
+                    ("vec_rank", ctypes.c_int32), ("dim_rank", ctypes.c_int32),
+                    ("dims", ctypes.POINTER(dim_t)), ("vecs", ctypes.POINTER(dim_t)),
+                    ("flags", Flags), ("pthr_fft", Smp), ("cntrl_params", Cntrl)]
+    return ProbDesc
+
+
+ProbDesc = _desc(Dim)
+ProbDesc64 = _desc(Dim64)
+
+# Every extern "C" symbol the library exports, as declared in include/*.h.
+EXPORTS = (
+    "aoclfftz_setup_f", "aoclfftz_setup_d", "aoclfftz_setup_f_64_", "aoclfftz_setup_d_64_",
+    "aoclfftz_execute", "aoclfftz_execute_io", "aoclfftz_destroy", "aoclfftz_version",
+    "aoclfftz_gpu_set_stream", "aoclfftz_gpu_synchronize", "aoclfftz_gpu_set_graph",
+    "aoclfftz_gpu_plan_info", "aoclfftz_gpu_plan_flops", "aoclfftz_gpu_plan_device_bytes",
+    "aoclfftz_gpu_plan_json", "aoclfftz_gpu_profile", "aoclfftz_gpu_profile_read",
+    "aoclfftz_gpu_plan_launches",
+)
+
+_lib = None
+
+
+def load():
+    """Load the built library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: run aocl-fftz_amd/build.py (there is no CPU fallback)")
+    lib = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    for name, dt in (("aoclfftz_setup_f", ProbDesc), ("aoclfftz_setup_d", ProbDesc),
+                     ("aoclfftz_setup_f_64_", ProbDesc64), ("aoclfftz_setup_d_64_", ProbDesc64)):
+        fn = getattr(lib, name)
+        fn.argtypes = [ctypes.POINTER(dt)]
+        fn.restype = vp
+    lib.aoclfftz_execute.argtypes = [vp]
+    lib.aoclfftz_execute.restype = ctypes.c_int
+    lib.aoclfftz_execute_io.argtypes = [vp, vp, vp]
+    lib.aoclfftz_execute_io.restype = ctypes.c_int
+    lib.aoclfftz_destroy.argtypes = [vp]
+    lib.aoclfftz_destroy.restype = None
+    lib.aoclfftz_version.restype = ctypes.c_char_p
+    lib.aoclfftz_gpu_set_stream.argtypes = [vp, vp]
+    lib.aoclfftz_gpu_set_stream.restype = ctypes.c_int
+    lib.aoclfftz_gpu_synchronize.argtypes = [vp]
+    lib.aoclfftz_gpu_synchronize.restype = ctypes.c_int
+    lib.aoclfftz_gpu_set_graph.argtypes = [vp, ctypes.c_int32]
+    lib.aoclfftz_gpu_set_graph.restype = ctypes.c_int
+    lib.aoclfftz_gpu_plan_info.argtypes = [vp]
+    lib.aoclfftz_gpu_plan_info.restype = ctypes.c_char_p
+    lib.aoclfftz_gpu_plan_flops.argtypes = [vp]
+    lib.aoclfftz_gpu_plan_flops.restype = ctypes.c_double
+    lib.aoclfftz_gpu_plan_json.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_char_p, ctypes.c_int64]
+    lib.aoclfftz_gpu_plan_json.restype = ctypes.c_int64
+    lib.aoclfftz_gpu_profile.argtypes = [vp, ctypes.c_int32]
+    lib.aoclfftz_gpu_profile.restype = ctypes.c_int
+    lib.aoclfftz_gpu_profile_read.argtypes = [vp, ctypes.POINTER(ctypes.c_float), ctypes.c_int32]
+    lib.aoclfftz_gpu_profile_read.restype = ctypes.c_int32
+    lib.aoclfftz_gpu_plan_launches.argtypes = [vp]
+    lib.aoclfftz_gpu_plan_launches.restype = ctypes.c_int32
+    lib.aoclfftz_gpu_plan_device_bytes.argtypes = [vp]
+    lib.aoclfftz_gpu_plan_device_bytes.restype = ctypes.c_int64
+    _lib = lib
+    return lib
+
+
+def _ptr(buf) -> int:
+    """Address of a torch tensor, numpy array, int, or None."""
+    if buf is None:
+        return 0
+    if isinstance(buf, int):
+        return buf
+    if hasattr(buf, "data_ptr"):
+        return buf.data_ptr()
+    if hasattr(buf, "ctypes"):
+        return buf.ctypes.data
+    raise TypeError(f"unsupported buffer {type(buf)}")
+
+
+def make_desc(dims: Sequence[Tuple[int, int, int]], vecs: Sequence[Tuple[int, int, int]], *,
+              inp=None, out=None, fft_type: int = 0, direction: int = 0, placement: Optional[int] = None,
+              ilp64: bool = False, storage_order: int = 0, transpose_mode: int = 0, bit_repro: int = 0,
+              threads: int = 1):
+    """Build a descriptor (and keep its arrays alive on the returned object)."""
+    dt, pt = (Dim64, ProbDesc64) if ilp64 else (Dim, ProbDesc)
+    d = pt()
+    darr = (dt * max(len(dims), 1))(*[dt(*x) for x in dims])
+    varr = (dt * max(len(vecs), 1))(*[dt(*x) for x in vecs])
+    d.in_ = _ptr(inp)
+    d.out = _ptr(out)
+    d.dim_rank = len(dims)
+    d.vec_rank = len(vecs)
+    d.dims = darr if dims else ctypes.POINTER(dt)()
+    d.vecs = varr if vecs else ctypes.POINTER(dt)()
+    if placement is None:
+        placement = 0 if (_ptr(inp) == _ptr(out) and _ptr(inp) != 0) else 1
+    d.flags = Flags(fft_type, direction, storage_order, placement, transpose_mode, bit_repro)
+    d.pthr_fft = Smp(threads, 1)
+    d.cntrl_params = Cntrl(3, 0, 0, 0)
+    d._keep = (darr, varr)
+    return d
+
+
+def setup(desc, precision: str, ilp64: bool = False):
+    lib = load()
+    fn = {("f", False): lib.aoclfftz_setup_f, ("d", False): lib.aoclfftz_setup_d,
+          ("f", True): lib.aoclfftz_setup_f_64_, ("d", True): lib.aoclfftz_setup_d_64_}[(precision, ilp64)]
+    return fn(ctypes.byref(desc))
+
+
+class Plan:
+    """Owning wrapper around one library handle."""
+
+    def __init__(self, dims, vecs, precision: str = "f", direction: int = 0, fft_type: int = 0,
+                 inplace: bool = False, inp=None, out=None, ilp64: bool = False):
+        self.precision = precision
+        placement = 0 if inplace else 1
+        if inp is None and out is None and not inplace:
+            # setup with NULL buffers: out-of-place with "exactly one NULL"
+            # is the documented way; use a dummy non-NULL marker for out.
+            inp, out = None, 1
+        self.desc = make_desc(dims, vecs, inp=inp, out=out, fft_type=fft_type, direction=direction,
+                              placement=placement, ilp64=ilp64)
+        self.handle = setup(self.desc, precision, ilp64)
+        if not self.handle:
+            raise RuntimeError("aoclfftz setup failed")
+
+    def execute(self) -> int:
+        return load().aoclfftz_execute(self.handle)
+
+    def execute_io(self, inp, out) -> int:
+        return load().aoclfftz_execute_io(self.handle, _ptr(inp), _ptr(out))
+
+    def set_stream(self, stream) -> int:
+        s = 0 if stream is None else (stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream))
+        return load().aoclfftz_gpu_set_stream(self.handle, s or None)
+
+    def synchronize(self) -> int:
+        return load().aoclfftz_gpu_synchronize(self.handle)
+
+    def set_graph(self, enable: bool) -> int:
+        return load().aoclfftz_gpu_set_graph(self.handle, 1 if enable else 0)
+
+    def profile(self, enable: bool) -> int:
+        return load().aoclfftz_gpu_profile(self.handle, 1 if enable else 0)
+
+    def profile_read(self):
+        """Per-launch kernel durations (ms) since the last read, as a list of
+        per-execute lists."""
+        lib = load()
+        cap = 1 << 16
+        buf = (ctypes.c_float * cap)()
+        n = lib.aoclfftz_gpu_profile_read(self.handle, buf, cap)
+        per = lib.aoclfftz_gpu_plan_launches(self.handle)
+        vals = list(buf[:min(n, cap)])
+        return [vals[i:i + per] for i in range(0, len(vals), per)]
+
+    @property
+    def info(self) -> str:
+        return load().aoclfftz_gpu_plan_info(self.handle).decode()
+
+    @property
+    def flops(self) -> float:
+        return load().aoclfftz_gpu_plan_flops(self.handle)
+
+    def destroy(self) -> None:
+        if self.handle:
+            load().aoclfftz_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def plan_json(dims, vecs, precision: str = "f", direction: int = 0, fft_type: int = 0,
+              inplace: bool = False) -> dict:
+    """Dry-run plan (no GPU needed): the launch list as a dict."""
+    import json
+    lib = load()
+    placement = 0 if inplace else 1
+    d = make_desc(dims, vecs, inp=0x1000, out=0x1000 if inplace else 0x2000, fft_type=fft_type,
+                  direction=direction, placement=placement, ilp64=True)
+    pb = 4 if precision == "f" else 8
+    n = lib.aoclfftz_gpu_plan_json(ctypes.byref(d), pb, 1, None, 0)
+    if n < 0:
+        raise ValueError("plan rejected")
+    buf = ctypes.create_string_buffer(int(n) + 1)
+    lib.aoclfftz_gpu_plan_json(ctypes.byref(d), pb, 1, buf, n + 1)
+    return json.loads(buf.value.decode())
+
+
+def version() -> str:
+    return load().aoclfftz_version().decode()

@@ -1,0 +1,299 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X FFT library on its headline workloads.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Primary workload (the `value` of the JSON line): the large single transform --
+1-D C2C complex128, N = 2^24, batch 1, in place, one forward + one backward
+transform per step (the four-step path), inputs i.i.d. normal(0,1) from seed 1
+already resident in HBM.  GFLOP/s counts 5 N log2 N per transform.
+
+Also measured in the same run and reported in the same JSON line:
+  * batched FP32 C2C, N = 1024, batch 4096, out of place (single LDS pass),
+    with input buffers rotated over 512 MiB so the 256 MiB Infinity Cache does
+    not serve them, plus the cache-warm figure;
+  * roofline of the dominant kernel: algorithmic bytes per launch over its
+    average duration measured with HIP events on its own stream;
+  * the reference's CPU path (AOCL-FFTZ built from its sources, OpenMP) timed
+    on this host on a bounded sample of the same workload, with the relative
+    L2 difference between the GPU and the reference on the same input.
+
+One GPU only: --gpus above 1 exits with a message.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "aocl-fftz_amd"))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def launch_bytes(plan_json):
+    """Algorithmic HBM bytes of each launch: every element read once and
+    written once."""
+    out = []
+    pb = plan_json["prec"]
+    for ln in plan_json["launches"]:
+        in_e = pb if ln["flags"] & 512 else 2 * pb
+        out_e = pb if ln["flags"] & 1024 else 2 * pb
+        nin = 2 * ln["L"] if ln["flags"] & 512 else ln["nvalid_in"]
+        nout = ln["L"] + 1 if ln["flags"] & 128 else ln["nvalid_out"]
+        if ln["flags"] & 1024:
+            nout = 2 * ln["L"]
+        out.append(ln["nlines"] * (nin * in_e + nout * out_e))
+    return out
+
+
+def roofline_from_profile(prof, bytes_per_launch):
+    """Average duration per launch index; the dominant kernel is the launch
+    index with the largest total time."""
+    import numpy as np
+    arr = np.array(prof, dtype=np.float64)  # executes x launches (ms)
+    avg = arr.mean(axis=0)
+    k = int(np.argmax(avg))
+    achieved = bytes_per_launch[k] / (avg[k] * 1e-3)
+    return k, avg, achieved
+
+
+def bench_roundtrip_2p24(torch, fftz, steps, warmup, n=1 << 24):
+    """N=2^24 complex128 in place, forward + backward per step."""
+    g = torch.Generator(device="cuda")
+    g.manual_seed(1)
+    x = torch.randn(n, dtype=torch.complex128, device="cuda", generator=g)
+    x0 = x.clone()
+    stream = torch.cuda.current_stream()
+    fwd = fftz.Plan([(n, 1, 1)], [(1, n, n)], "d", 0, inplace=True, inp=x, out=x)
+    bwd = fftz.Plan([(n, 1, 1)], [(1, n, n)], "d", 1, inplace=True, inp=x, out=x)
+    for p in (fwd, bwd):
+        assert p.set_stream(stream) == 0
+    pj = fftz.plan_json([(n, 1, 1)], [(1, n, n)], "d", 0, inplace=True)
+    log(f"[bench] 2^24 fp64 plan: {fwd.info}")
+    for _ in range(warmup):
+        fwd.execute()
+        bwd.execute()
+    torch.cuda.synchronize()
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    fwd.profile(True)
+    bwd.profile(True)
+    torch.cuda.synchronize()
+    t0.record(stream)
+    for _ in range(steps):
+        assert fwd.execute() == 0
+        assert bwd.execute() == 0
+    t1.record(stream)
+    torch.cuda.synchronize()
+    ms = t0.elapsed_time(t1) / steps
+    prof_f = fwd.profile_read()
+    prof_b = bwd.profile_read()
+    fwd.profile(False)
+    bwd.profile(False)
+    # round trip: x was scaled by N per step; N = 2^24 so the rescale is exact
+    k = steps + warmup
+    back = x / (float(n) ** k)
+    rt_err = (torch.linalg.vector_norm(back - x0) / torch.linalg.vector_norm(x0)).item()
+    bpl = launch_bytes(pj)
+    kidx, avg, achieved = roofline_from_profile(prof_f + prof_b, bpl)
+    flops = 2 * 5.0 * n * math.log2(n)
+    res = {
+        "ms_per_step": ms,
+        "gflops": flops / (ms * 1e-3) / 1e9,
+        "transforms_per_s": 2.0 / (ms * 1e-3),
+        "roundtrip_rel_l2": rt_err,
+        "launch_ms_avg": [round(v, 4) for v in avg.tolist()],
+        "launch_bytes": bpl,
+        "dominant_launch": kidx,
+        "dominant_achieved_Bps": achieved,
+        "transform_frac_of_hbm": (2 * 16.0 * n * 2) / (ms * 1e-3) / HBM_PEAK,
+        "plan": fwd.info,
+    }
+    # keep the forward input/output of one fresh transform for the reference
+    # comparison in the CPU-baseline leg
+    xin = x0.clone()
+    fwd2 = fftz.Plan([(n, 1, 1)], [(1, n, n)], "d", 0, inplace=True, inp=xin, out=xin)
+    fwd2.set_stream(stream)
+    fwd2.execute()
+    torch.cuda.synchronize()
+    res["_x0"] = x0.cpu().numpy()
+    res["_y"] = xin.cpu().numpy()
+    for p in (fwd, bwd, fwd2):
+        p.destroy()
+    del x, x0, xin, back
+    torch.cuda.empty_cache()
+    return res
+
+
+def bench_batched_n1024(torch, fftz, steps, warmup, n=1024, b=4096):
+    """N=1024 complex64, batch 4096, out of place."""
+    g = torch.Generator(device="cuda")
+    g.manual_seed(1)
+    nbuf = 8  # 8 x (32 + 32) MiB = 512 MiB > Infinity Cache
+    xs = [(torch.rand(b, n, dtype=torch.complex64, device="cuda", generator=g) * 2 - (1 + 1j)) for _ in range(nbuf)]
+    ys = [torch.empty_like(xs[0]) for _ in range(nbuf)]
+    stream = torch.cuda.current_stream()
+    p = fftz.Plan([(n, 1, 1)], [(b, n, n)], "f", 0)
+    p.set_stream(stream)
+    pj = fftz.plan_json([(n, 1, 1)], [(b, n, n)], "f", 0)
+    bpl = launch_bytes(pj)
+    log(f"[bench] n1024 fp32 plan: {p.info}")
+    for i in range(warmup * nbuf):
+        p.execute_io(xs[i % nbuf], ys[i % nbuf])
+    torch.cuda.synchronize()
+    reps = max(steps, 1) * 20
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    p.profile(True)
+    t0.record(stream)
+    for i in range(reps):
+        p.execute_io(xs[i % nbuf], ys[i % nbuf])
+    t1.record(stream)
+    torch.cuda.synchronize()
+    ms_cold = t0.elapsed_time(t1) / reps
+    prof = p.profile_read()
+    p.profile(False)
+    # cache-warm: the same buffers every time
+    t0.record(stream)
+    for i in range(reps):
+        p.execute_io(xs[0], ys[0])
+    t1.record(stream)
+    torch.cuda.synchronize()
+    ms_warm = t0.elapsed_time(t1) / reps
+    _, avg, achieved = roofline_from_profile(prof, bpl)
+    flops = 5.0 * n * math.log2(n) * b
+    byts = 2 * 8.0 * n * b
+    res = {
+        "ms_per_execute": ms_cold,
+        "gflops": flops / (ms_cold * 1e-3) / 1e9,
+        "transforms_per_s": b / (ms_cold * 1e-3),
+        "frac_of_hbm": byts / (ms_cold * 1e-3) / HBM_PEAK,
+        "kernel_ms_avg": float(avg[0]),
+        "kernel_achieved_Bps": achieved,
+        "ms_per_execute_cache_warm": ms_warm,
+        "gflops_cache_warm": flops / (ms_warm * 1e-3) / 1e9,
+        "plan": p.info,
+    }
+    p.execute_io(xs[0], ys[0])
+    torch.cuda.synchronize()
+    res["_x0"] = xs[0].cpu().numpy()
+    res["_y"] = ys[0].cpu().numpy()
+    p.destroy()
+    del xs, ys
+    torch.cuda.empty_cache()
+    return res
+
+
+def cpu_baseline(res_2p24, res_1024, budget_s):
+    """The reference's OpenMP CPU path, timed on this host (bounded sample)."""
+    import numpy as np
+    from oracle import reflib
+    if not reflib.available():
+        return None, {}
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    cores = max(1, min(cores, os.cpu_count() or 1))
+    n = 1 << 24
+    x = np.ascontiguousarray(res_2p24["_x0"])
+    buf = x.copy()
+    plan = reflib.RefPlan([(n, 1, 1)], [(1, n, n)], "d", -1, buf, buf, threads=cores)
+    t_end = time.perf_counter() + budget_s
+    times = []
+    first = True
+    while time.perf_counter() < t_end or len(times) < 2:
+        buf[:] = x
+        t0 = time.perf_counter()
+        plan.execute()
+        times.append(time.perf_counter() - t0)
+        if first:
+            ref_y = buf.copy()
+            first = False
+    plan.destroy()
+    t = min(times)
+    gflops = 5.0 * n * math.log2(n) / t / 1e9
+    y = res_2p24["_y"]
+    rel = float(np.linalg.norm(y - ref_y) / np.linalg.norm(ref_y))
+    extra = {"rel_l2_vs_reference_2p24": rel}
+    # batched N=1024 vs the reference's FP64 transform of the upcast input
+    xb = res_1024["_x0"].astype(np.complex128)
+    yb = reflib.c2c(xb[:256], -1, "d", threads=cores)
+    extra["rel_l2_vs_reference_n1024"] = float(np.linalg.norm(res_1024["_y"][:256] - yb) / np.linalg.norm(yb))
+    t_f, reps = reflib.time_c2c(1024, 4096, "f", cores, budget_s=min(5.0, budget_s / 2))
+    extra["cpu_n1024_gflops"] = 5.0 * 1024 * 10 * 4096 / t_f / 1e9
+    base = {"value": gflops, "unit": "GFLOP/s", "cores": cores, "kind": "reference",
+            "sample": f"{len(times)} forward executes of the N=2^24 complex128 in-place transform "
+                      f"(min time {t * 1e3:.1f} ms), AOCL-FFTZ built from its sources, OpenMP {cores} threads; "
+                      f"N=1024 x 4096 complex64: {reps} executes"}
+    return base, extra
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the transforms here "
+              "are single-GPU by design and no multi-GPU figure is produced.", file=sys.stderr)
+        sys.exit(2)
+    if args.steps + args.warmup > 40:
+        sys.exit("steps + warmup must stay <= 40 (unnormalised round trips grow by N per step)")
+    import torch
+    import fftz
+    assert torch.cuda.is_available()
+    fftz.load()
+    log("[bench] start 2^24 fp64 round trip")
+    r24 = bench_roundtrip_2p24(torch, fftz, args.steps, args.warmup)
+    log(f"[bench] 2^24: {r24['ms_per_step']:.3f} ms/step {r24['gflops']:.1f} GFLOP/s")
+    log("[bench] start N=1024 batch 4096 fp32")
+    r10 = bench_batched_n1024(torch, fftz, args.steps, args.warmup)
+    log(f"[bench] n1024: {r10['ms_per_execute'] * 1e3:.2f} us {r10['gflops']:.1f} GFLOP/s "
+        f"frac {r10['frac_of_hbm']:.3f}")
+    base, extra = (None, {})
+    if not args.no_cpu:
+        log("[bench] cpu baseline (reference, bounded sample)")
+        base, extra = cpu_baseline(r24, r10, args.cpu_budget)
+    achieved = r24["dominant_achieved_Bps"]
+    line = {
+        "metric": "C2C FFT GFLOP/s (5Nlog2N) + %HBM roofline; MFMA-split vs plain at same error",
+        "value": round(r24["gflops"], 2),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r24["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: i.i.d. normal(0,1) complex128 from seed 1, resident in HBM",
+        "config": {"workload": "1-D C2C complex128 N=2^24 batch=1 in-place, forward+backward per step "
+                               "(four-step path)", "n": 1 << 24, "batch": 1, "transforms_per_step": 2},
+        "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4),
+                     "traffic": None,
+                     "kernel": f"k_fft_pass launch #{r24['dominant_launch']} of the four-step plan",
+                     "algorithmic_bytes_per_launch": r24["launch_bytes"][r24["dominant_launch"]]},
+        "cpu_baseline": base,
+        "roundtrip_rel_l2": r24["roundtrip_rel_l2"],
+        "transform_frac_of_hbm": round(r24["transform_frac_of_hbm"], 4),
+        "launch_ms_avg": r24["launch_ms_avg"],
+        "batched_n1024_fp32": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r10.items()
+                               if not k.startswith("_")},
+    }
+    line.update(extra)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,68 @@
+/* aoclfftz_gpu.h -- GPU-only extensions of the MI355X FFT library.
+ *
+ * None of these has a counterpart in the reference (a CPU library has no
+ * streams or device buffers).  They are for callers that keep data resident
+ * in HBM and want asynchronous execution; a program that only uses
+ * aoclfftz.h never needs them.
+ */
+#ifndef AOCLFFTZ_GPU_H
+#define AOCLFFTZ_GPU_H
+
+#include "aoclfftz.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Bind a HIP stream (hipStream_t passed as void*) to the handle.  Afterwards
+ * aoclfftz_execute / aoclfftz_execute_io on DEVICE buffers enqueue on that
+ * stream and return without waiting; host buffers still run synchronously.
+ * Passing NULL restores the default synchronous behaviour.  Calls that share
+ * one handle in stream mode are ordered by the stream; do not use one handle
+ * from several streams at once. */
+aoclfftz_error_type aoclfftz_gpu_set_stream(FFTZ_VOID *handle, FFTZ_VOID *stream);
+
+/* Wait for all work the handle has enqueued. */
+aoclfftz_error_type aoclfftz_gpu_synchronize(FFTZ_VOID *handle);
+
+/* Capture the handle's launches into one hipGraph per (in, out) pair and
+ * replay it (enable = 1), or launch kernels directly (enable = 0). */
+aoclfftz_error_type aoclfftz_gpu_set_graph(FFTZ_VOID *handle, FFTZ_INT32 enable);
+
+/* Human-readable plan summary (pass list with lengths, radices, tiles).  The
+ * string lives as long as the handle. */
+const FFTZ_CHAR *aoclfftz_gpu_plan_info(FFTZ_VOID *handle);
+
+/* Algorithmic flop count of one execute: 5 N log2 N per complex transform,
+ * 2.5 N log2 N per real transform, times the batch. */
+FFTZ_DOUBLE aoclfftz_gpu_plan_flops(FFTZ_VOID *handle);
+
+/* Device bytes held by the plan (tables) and per execution slot (work
+ * buffer). */
+FFTZ_INT64 aoclfftz_gpu_plan_device_bytes(FFTZ_VOID *handle);
+
+/* Per-launch timing.  While enabled, every execute records a HIP event
+ * before its first kernel and after each kernel, on the stream the kernels
+ * run on (direct launches; the graph path is bypassed).  profile_read waits
+ * for those events and writes the kernel durations in milliseconds, in
+ * launch order, execute after execute; it returns how many were recorded
+ * (which may exceed cap) and recycles the events. */
+aoclfftz_error_type aoclfftz_gpu_profile(FFTZ_VOID *handle, FFTZ_INT32 enable);
+FFTZ_INT32 aoclfftz_gpu_profile_read(FFTZ_VOID *handle, float *ms, FFTZ_INT32 cap);
+
+/* Number of kernel launches per execute. */
+FFTZ_INT32 aoclfftz_gpu_plan_launches(FFTZ_VOID *handle);
+
+/* Diagnostic: validate a descriptor (aoclfftz_prob_desc_f/_d, or the _64_
+ * variants when ilp64 != 0; the buffer type does not matter), plan it
+ * without touching the device and write a JSON description of every launch
+ * (lengths, strides, line geometry, flags) into buf.  Returns the full JSON
+ * length, or -1 on an invalid / unsupported problem.  Needs no GPU. */
+FFTZ_INT64 aoclfftz_gpu_plan_json(FFTZ_VOID *problem, FFTZ_INT32 precision_bytes, FFTZ_INT32 ilp64,
+                                  FFTZ_CHAR *buf, FFTZ_INT64 buflen);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AOCLFFTZ_GPU_H */

@@ -1,0 +1,137 @@
+"""CPU emulation of the library's launch list (numpy), for planner checks.
+
+aoclfftz_gpu_plan_json gives every launch's geometry without a GPU.  This
+module replays those launches with the semantics of kernels.hip -- tile
+gather with the line decomposition l = l0 + ln0*(l1 + ln1*l2), conjugation,
+the L-point transform, four-step twiddle, real-packed recombine/separate,
+Bluestein -- on flat numpy buffers sized exactly like the caller's and the
+work buffer, and asserts every gathered / scattered index is in bounds.
+A planner geometry bug therefore shows up here, on the CPU, as an
+out-of-bounds index or a wrong result, before any GPU run.
+"""
+import numpy as np
+
+K_CONJ_IN, K_CONJ_OUT, K_TW4 = 1, 2, 4
+K_SCALE, K_R2C_POST, K_C2R_PRE, K_REAL_IN, K_REAL_OUT = 32, 128, 256, 512, 1024
+
+
+class OOB(AssertionError):
+    pass
+
+
+def _line_offsets(ln, nl):
+    l = np.arange(nl, dtype=np.int64)
+    l0 = l % ln["ln0"]
+    r = l // ln["ln0"]
+    l1 = r % ln["ln1"]
+    l2 = r // ln["ln1"]
+    ioff = l0 * ln["ils"][0] + l1 * ln["ils"][1] + l2 * ln["ils"][2]
+    ooff = l0 * ln["ols"][0] + l1 * ln["ols"][1] + l2 * ln["ols"][2]
+    return ioff, ooff, l0
+
+
+def _check(idx, size, what):
+    if idx.size and (idx.min() < 0 or idx.max() >= size):
+        raise OOB(f"{what}: index range [{idx.min()}, {idx.max()}] outside buffer of {size}")
+
+
+def run(plan, bufs):
+    """bufs: dict role -> flat numpy array (complex128 or float64 views).
+    Roles: 0 in, 1 out, 2 work.  Arrays are modified in place."""
+    for ln in plan["launches"]:
+        src = bufs[ln["in_role"]]
+        dst = bufs[ln["out_role"]]
+        L = ln["L"]
+        flags = ln["flags"]
+        nl = ln["nlines"]
+        ioff, ooff, l0 = _line_offsets(ln, nl)
+        in_es = src.itemsize
+        out_es = dst.itemsize
+        assert ln["in_off"] % in_es == 0 and ln["out_off"] % out_es == 0
+        ioff = ioff + ln["in_off"] // in_es
+        ooff = ooff + ln["out_off"] // out_es
+        if flags & K_REAL_IN:
+            j = np.arange(L, dtype=np.int64)
+            ire = ioff[:, None] + (2 * j)[None, :] * ln["ies"]
+            iim = ioff[:, None] + (2 * j + 1)[None, :] * ln["ies"]
+            _check(ire, src.size, "real load")
+            _check(iim, src.size, "real load")
+            x = src[ire] + 1j * src[iim]
+        else:
+            nv = ln["nvalid_in"]
+            e = np.arange(nv, dtype=np.int64)
+            idx = ioff[:, None] + e[None, :] * ln["ies"]
+            _check(idx, src.size, "load")
+            x = np.zeros((nl, max(L, nv)), np.complex128)
+            x[:, :nv] = src[idx]
+        if flags & K_CONJ_IN:
+            x = np.conj(x)
+        if ln["kind"] == "bluestein":
+            n = ln["nvalid_in"]
+            y = np.fft.fft(x[:, :n], axis=1)
+            if flags & K_CONJ_OUT:
+                y = np.conj(y)
+            e = np.arange(n, dtype=np.int64)
+            idx = ooff[:, None] + e[None, :] * ln["oes"]
+            _check(idx, dst.size, "bluestein store")
+            dst[idx] = y
+            continue
+        if flags & K_C2R_PRE:
+            X = x[:, :L + 1]
+            k = np.arange(L)
+            Xm = np.conj(X[:, L - k])
+            w = np.exp(2j * np.pi * k / (2 * L))
+            Z = (X[:, :L] + Xm) + 1j * ((X[:, :L] - Xm) * w)
+            x = np.conj(Z)
+        y = np.fft.fft(x[:, :L], axis=1)
+        if flags & K_R2C_POST:
+            Z = y
+            k = np.arange(L + 1)
+            Zk = Z[:, k % L]
+            Zm = np.conj(Z[:, (L - k) % L])
+            w = np.exp(-2j * np.pi * k / (2 * L))
+            X = 0.5 * (Zk + Zm) + w * (Zk - Zm) / 2j
+            idx = ooff[:, None] + k[None, :] * ln["oes"]
+            _check(idx, dst.size, "recombine store")
+            dst[idx] = X
+            continue
+        nvo = ln["nvalid_out"]
+        y = y[:, :nvo]
+        e = np.arange(nvo, dtype=np.int64)
+        if flags & K_TW4:
+            N = ln["tw4_n"]
+            p = (l0[:, None] * e[None, :])
+            assert p.max() < N, "four-step twiddle exponent exceeds its table"
+            y = y * np.exp(-2j * np.pi * p / N)
+        if flags & K_CONJ_OUT:
+            y = np.conj(y)
+        if flags & K_REAL_OUT:
+            ire = ooff[:, None] + (2 * e)[None, :] * ln["oes"]
+            iim = ooff[:, None] + (2 * e + 1)[None, :] * ln["oes"]
+            _check(ire, dst.size, "real store")
+            _check(iim, dst.size, "real store")
+            dst[ire] = y.real
+            dst[iim] = y.imag
+        else:
+            idx = ooff[:, None] + e[None, :] * ln["oes"]
+            _check(idx, dst.size, "store")
+            dst[idx] = y
+    return bufs
+
+
+def check_launch_limits(plan, lds_cap=163840):
+    """Launch-shape invariants the kernels rely on."""
+    for ln in plan["launches"]:
+        L, C, TL, E = ln["L"], ln["C"], ln["TL"], ln["E"]
+        assert ln["block"] == C * TL <= 1024
+        assert C & (C - 1) == 0 and TL & (TL - 1) == 0
+        assert TL * E >= L, "threads x elements must cover the line"
+        for r in ln["radix"]:
+            K = -(-E // r)
+            assert TL * K >= L // r, "stage butterflies not covered"
+        assert ln["lds"] <= lds_cap
+        need = L + (1 if ln["flags"] & K_C2R_PRE else 0)
+        assert ln["LP"] >= (need - 1) + ((need - 1) >> 4) + 1
+        assert ln["lds"] == C * ln["LP"] * (16 if plan["prec"] == 8 else 8)
+        assert ln["grid"] * C >= ln["nlines"]
+        assert ln["nlines"] < 2 ** 31

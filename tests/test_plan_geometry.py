@@ -1,0 +1,155 @@
+"""Planner geometry checked on the CPU (no GPU): every plan's launch list is
+replayed by tests/plan_emulator.py with bounds checks on each access and the
+result compared with the oracle."""
+import numpy as np
+import pytest
+
+import fftz
+import plan_emulator as emu
+from oracle import oracle as O
+
+
+def rnd(n, seed):
+    rng = np.random.default_rng(seed)
+    return rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)
+
+
+def c2c_case(dims, vecs, prec="f", direction=0, inplace=False):
+    plan = fftz.plan_json(dims, vecs, prec, direction, inplace=inplace)
+    emu.check_launch_limits(plan)
+    n_in = plan["in_extent_bytes"] // (2 * plan["prec"])
+    n_out = plan["out_extent_bytes"] // (2 * plan["prec"])
+    x = rnd(n_in, 5)
+    out = x.copy() if inplace else np.zeros(n_out, np.complex128)
+    work = np.zeros(max(1, plan["work_bytes"] // (2 * plan["prec"])), np.complex128)
+    bufs = {0: x.copy() if not inplace else out, 1: out, 2: work}
+    emu.run(plan, bufs)
+    ref = np.zeros(n_out, np.complex128) if not inplace else x.copy()
+    O.c2c_strided(x.copy(), ref, dims, vecs, -1 if direction == 0 else 1)
+    err = O.rel_l2(bufs[1], ref)
+    assert err < 1e-12, (err, plan)
+    return plan
+
+
+@pytest.mark.parametrize("n", [1, 2, 16, 17, 64, 100, 101, 1024, 4096, 8191, 16384, 12288])
+@pytest.mark.parametrize("prec", ["f", "d"])
+def test_1d_batched(n, prec):
+    if prec == "d" and n == 8191:
+        pytest.skip("FP64 Bluestein above single-pass limit")
+    b = 3
+    c2c_case([(n, 1, 1)], [(b, n, n)], prec)
+
+
+@pytest.mark.parametrize("n,prec", [(1 << 15, "f"), (1 << 17, "d"), (1 << 20, "f"), (1 << 20, "d"),
+                                    (3 * (1 << 14), "f"), (30030, "f"), (30030, "d"), (16384, "d"),
+                                    (1 << 22, "d")])
+def test_fourstep(n, prec):
+    plan = c2c_case([(n, 1, 1)], [(2, n, n)], prec)
+    assert len(plan["launches"]) >= 2
+
+
+def test_fourstep_cfg2_shapes():
+    # the large single transforms: 2^24 and 2^20*3, FP64, in-place, batch 1
+    for n in (1 << 24, 3 * (1 << 20)):
+        plan = fftz.plan_json([(n, 1, 1)], [(1, n, n)], "d", 0, inplace=True)
+        emu.check_launch_limits(plan)
+        assert 2 <= len(plan["launches"]) <= 3
+
+
+def test_fourstep_2_24_emulated():
+    n = 1 << 24
+    plan = c2c_case([(n, 1, 1)], [(1, n, n)], "d", inplace=True)
+    assert len(plan["launches"]) == 3
+
+
+@pytest.mark.parametrize("direction", [0, 1])
+def test_strided_inplace_backward(direction):
+    c2c_case([(48, 3, 3)], [(5, 1, 1)], "f", direction, inplace=True)
+    c2c_case([(1 << 15, 2, 2)], [(2, 1, 1)], "f", direction, inplace=True)
+
+
+@pytest.mark.parametrize("shape", [(16, 16), (64, 48), (8, 12, 10), (101, 8), (3, 5, 7, 4), (128, 128, 128)])
+def test_multidim(shape):
+    dims = []
+    stride = 1
+    for n in reversed(shape):
+        dims.append((n, stride, stride))
+        stride *= n
+    c2c_case(dims, [(2, stride, stride)], "f")
+
+
+def test_multi_vec():
+    c2c_case([(64, 1, 1)], [(4, 64, 64), (3, 320, 256)], "f")
+    c2c_case([(12, 37, 37)], [(37, 1, 1)], "d")
+
+
+def real_case(n0_shape, prec, batch=2, inplace=False):
+    # row-major real input (padded in place), half-complex output
+    shape = n0_shape
+    n0 = shape[-1]
+    h = n0 // 2 + 1
+    dims_r2c = []
+    s_in = 2 * h if inplace else n0
+    s_out = h
+    dims_r2c.append((n0, 1, 1))
+    for n in reversed(shape[:-1]):
+        dims_r2c.append((n, s_in, s_out))
+        s_in *= n
+        s_out *= n
+    vecs = [(batch, s_in, s_out)]
+    plan = fftz.plan_json(dims_r2c, vecs, prec, 0, fft_type=1, inplace=inplace)
+    emu.check_launch_limits(plan)
+    rng = np.random.default_rng(3)
+    real_len = plan["in_extent_bytes"] // plan["prec"]
+    xr = rng.standard_normal(real_len)
+    out = np.zeros(plan["out_extent_bytes"] // (2 * plan["prec"]), np.complex128)
+    work = np.zeros(max(1, plan["work_bytes"] // (2 * plan["prec"])), np.complex128)
+    emu.run(plan, {0: xr, 1: out, 2: work})
+    # reference: per batch, dense real array -> rfftn
+    for b in range(batch):
+        xb = np.zeros(shape)
+        ob = np.zeros(shape[:-1] + (h,), complex)
+        for idx in np.ndindex(*shape):
+            off = b * vecs[0][1]
+            for ax, i in enumerate(reversed(idx)):
+                off += i * dims_r2c[ax][1]
+            xb[idx] = xr[off]
+        for idx in np.ndindex(*(shape[:-1] + (h,))):
+            off = b * vecs[0][2]
+            for ax, i in enumerate(reversed(idx)):
+                off += i * dims_r2c[ax][2]
+            ob[idx] = out[off]
+        assert O.rel_l2(ob, np.fft.rfftn(xb)) < 1e-12
+    # C2R back
+    plan_b = fftz.plan_json([(n0, d[2], d[1]) if i == 0 else (d[0], d[2], d[1]) for i, d in enumerate(dims_r2c)],
+                            [(batch, vecs[0][2], vecs[0][1])], prec, 1, fft_type=1, inplace=inplace)
+    emu.check_launch_limits(plan_b)
+    back = np.zeros(plan_b["out_extent_bytes"] // plan_b["prec"])
+    work = np.zeros(max(1, plan_b["work_bytes"] // (2 * plan_b["prec"])), np.complex128)
+    emu.run(plan_b, {0: out.copy(), 1: back, 2: work})
+    nt = int(np.prod(shape))
+    for b in range(batch):
+        for idx in list(np.ndindex(*shape))[:50]:
+            off = b * vecs[0][1]
+            for ax, i in enumerate(reversed(idx)):
+                off += i * dims_r2c[ax][1]
+            assert abs(back[off] / nt - xr[off]) < 1e-10
+
+
+@pytest.mark.parametrize("shape", [(16,), (1000,), (8, 16), (12, 10, 20)])
+def test_real_packed(shape):
+    real_case(shape, "f")
+
+
+def test_real_packed_inplace():
+    real_case((6, 32), "d", inplace=True)
+
+
+def test_cfg4a_plan_shape():
+    # 512^3 R2C: one real-packed pass plus two column passes
+    n = 512
+    h = n // 2 + 1
+    dims = [(n, 1, 1), (n, n, h), (n, n * n, n * h)]
+    plan = fftz.plan_json(dims, [(1, n ** 3, n * n * h)], "f", 0, fft_type=1)
+    emu.check_launch_limits(plan)
+    assert len(plan["launches"]) == 3
