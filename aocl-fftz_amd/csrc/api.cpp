@@ -255,10 +255,10 @@ aoclfftz_error_type run(Handle* h, void* in, void* out, const char* fname)
         report_error(fname, "in-place plan executed with distinct buffers");
         return AOCLFFTZ_EXECUTION_FAILURE;
     }
-    const bool in_dev = is_device_ptr(in);
-    const bool out_dev = is_device_ptr(out);
     hipError_t e;
-    if (in_dev && out_dev && h->stream) {
+    if (h->stream) {
+        // stream mode: the caller guarantees device buffers (aoclfftz_gpu.h);
+        // no pointer queries, nothing waits.
         std::lock_guard<std::mutex> g(h->amu);
         if (!h->aslot) {
             h->aslot = fftz::plan_acquire_slot(p, &e);
@@ -274,6 +274,8 @@ aoclfftz_error_type run(Handle* h, void* in, void* out, const char* fname)
         }
         return AOCLFFTZ_SUCCESS;
     }
+    const bool in_dev = is_device_ptr(in);
+    const bool out_dev = is_device_ptr(out);
     fftz::Slot* s = fftz::plan_acquire_slot(p, &e);
     if (!s) {
         report_error(fname, "slot allocation failed: %s", hipGetErrorString(e));
@@ -448,7 +450,8 @@ const FFTZ_CHAR* aoclfftz_gpu_plan_info(FFTZ_VOID* handle)
     fftz::Plan* p = h->plan;
     os << (p->spec.prec_bytes == 4 ? "fp32" : "fp64") << " " << p->description << "passes=" << p->launches.size();
     for (const auto& ln : p->launches) {
-        os << " [" << (ln.kind == fftz::KernelKind::Bluestein ? "bluestein" : "pass") << " L=" << ln.d.L << " r=";
+        os << " [" << (ln.kind == fftz::KernelKind::Bluestein ? "bluestein" : (ln.fixed ? "fixed" : "pass"))
+           << " L=" << ln.d.L << " r=";
         for (int s = 0; s < ln.d.nst; s++) os << (s ? "x" : "") << ln.d.radix[s];
         os << " C=" << ln.d.C << " TL=" << ln.d.TL << " grid=" << ln.grid << " block=" << ln.block
            << " lds=" << ln.lds_bytes << ((ln.d.flags & fftz::kInCol) ? " incol" : "")
@@ -487,6 +490,19 @@ FFTZ_INT64 aoclfftz_gpu_plan_json(FFTZ_VOID* problem, FFTZ_INT32 precision_bytes
         buf[n] = 0;
     }
     return (FFTZ_INT64)js.size();
+}
+
+aoclfftz_error_type aoclfftz_gpu_execute_many(FFTZ_VOID* handle, FFTZ_VOID** ins, FFTZ_VOID** outs,
+                                             FFTZ_INT32 count)
+{
+    Handle* h = as_handle(handle);
+    if (!h || count < 0 || (count > 0 && (!ins || !outs))) return AOCLFFTZ_EXECUTION_FAILURE;
+    for (int i = 0; i < count; i++) {
+        if (!ins[i] || !outs[i]) return AOCLFFTZ_EXECUTION_FAILURE;
+        aoclfftz_error_type rc = run(h, ins[i], outs[i], "aoclfftz_gpu_execute_many");
+        if (rc != AOCLFFTZ_SUCCESS) return rc;
+    }
+    return AOCLFFTZ_SUCCESS;
 }
 
 aoclfftz_error_type aoclfftz_gpu_profile(FFTZ_VOID* handle, FFTZ_INT32 enable)

@@ -13,4 +13,8 @@ enum class KernelKind { Pass, Bluestein };
 // 1024); the 256 variants get a larger VGPR budget.
 const void* kernel_symbol(KernelKind kind, int prec_bytes, bool all_radix, int E, int maxt);
 
+// Fixed-length pass kernel for power-of-two L in [64, 4096] (radix plan 16s
+// then remainder, 16 elements per thread, block <= 256); nullptr otherwise.
+const void* kernel_symbol_fixed(int prec_bytes, int L);
+
 }  // namespace fftz

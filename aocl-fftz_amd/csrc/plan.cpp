@@ -55,6 +55,10 @@ const DeviceInfo& device_info()
                             (void)hipFuncSetAttribute(kernel_symbol(k, prec, all, e, t),
                                                       hipFuncAttributeMaxDynamicSharedMemorySize,
                                                       (int)info.lds_per_block);
+        for (int prec : {4, 8})
+            for (int L = 64; L <= 4096; L *= 2)
+                (void)hipFuncSetAttribute(kernel_symbol_fixed(prec, L), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          (int)info.lds_per_block);
     });
     return info;
 }
@@ -342,6 +346,21 @@ static bool build_pass(Plan* plan, TableBuilder& tb, const PassReq& q, std::stri
     if ((LP & 1) == 0) LP++;
     d.LP = LP;
 
+    // fixed-length kernel: power-of-two L in [64, 4096], plain C2C pass
+    const bool pow2L = q.L >= 64 && q.L <= 4096 && (q.L & (q.L - 1)) == 0;
+    const bool fixed_ok = q.kind == KernelKind::Pass && pow2L && E == 16 && d.nvalid_in == q.L &&
+                          d.nvalid_out == q.L &&
+                          !(q.flags & (kRealIn | kRealOut | kR2CPost | kC2RPre)) && env_int("FFTZ_FIXED", 1);
+    bool plan_match = fixed_ok;
+    if (fixed_ok) {
+        // the fixed kernels hard-code 16, 16, ..., 2^rem (kernels_fixed.hip)
+        const int lg = ilog2(q.L), S = (lg + 3) / 4, rem = lg - 4 * (S - 1);
+        plan_match = (int)radix.size() == S;
+        for (int s = 0; plan_match && s < S; s++)
+            plan_match = radix[s] == (s < S - 1 ? 16 : (1 << rem));
+    }
+    const int tmax = plan_match ? 256 : 1024;
+
     // line axes: choose l0
     std::vector<LineAxis> ax = merge_axes(q.lines);
     if (ax.empty()) ax.push_back({1, 0, 0});
@@ -377,7 +396,7 @@ static bool build_pass(Plan* plan, TableBuilder& tb, const PassReq& q, std::stri
         const int cmax = env_int("FFTZ_COL_C", prec == 8 ? 16 : 32);
         const size_t budget = std::min(lds_cap, (size_t)env_int("FFTZ_COL_LDS", 96 * 1024));
         while (C * 2 <= cmax && (size_t)(C * 2) * line_bytes <= budget && (n0 % (C * 2)) == 0 &&
-               C * 2 * TL <= 1024)
+               C * 2 * TL <= tmax)
             C *= 2;
         if (n0 % C != 0 || C < 2) { in_col = false; out_col = false; C = 1; }
     }
@@ -386,7 +405,7 @@ static bool build_pass(Plan* plan, TableBuilder& tb, const PassReq& q, std::stri
         const size_t budget = std::min(lds_cap, (size_t)env_int("FFTZ_ROW_LDS", 64 * 1024));
         int64_t nl_total = 1;
         for (auto& a : ax) nl_total *= a.n;
-        while (C * TL < tgt && (size_t)(C * 2) * line_bytes <= budget && C * 2 * TL <= 1024 && C < nl_total) C *= 2;
+        while (C * TL < tgt && (size_t)(C * 2) * line_bytes <= budget && C * 2 * TL <= tmax && C < nl_total) C *= 2;
     }
     if (C * TL > 1024) {
         err = "tile too large";
@@ -427,6 +446,7 @@ static bool build_pass(Plan* plan, TableBuilder& tb, const PassReq& q, std::stri
     ln.out_role = q.out_role;
     ln.block = (uint32_t)(C * TL);
     ln.maxt = ln.block <= 256 ? 256 : 1024;
+    ln.fixed = plan_match && ln.block <= 256;
     const int64_t tiles = (nl + C - 1) / C;
     ln.grid = (uint32_t)tiles;
     ln.lds_bytes = (uint32_t)((size_t)C * line_bytes);
@@ -873,7 +893,7 @@ std::string plan_json(const Plan* p)
            << "," << d.ils[2] << "],\"ols\":[" << d.ols[0] << "," << d.ols[1] << "," << d.ols[2]
            << "],\"nlines\":" << d.nlines << ",\"ln0\":" << d.ln0.d << ",\"ln1\":" << d.ln1.d
            << ",\"flags\":" << d.flags << ",\"C\":" << d.C << ",\"TL\":" << d.TL << ",\"LP\":" << d.LP
-           << ",\"E\":" << ln.E << ",\"grid\":" << ln.grid << ",\"block\":" << ln.block
+           << ",\"E\":" << ln.E << ",\"fixed\":" << (ln.fixed ? 1 : 0) << ",\"grid\":" << ln.grid << ",\"block\":" << ln.block
            << ",\"lds\":" << ln.lds_bytes << ",\"in_role\":" << (int)ln.in_role
            << ",\"out_role\":" << (int)ln.out_role << ",\"in_off\":" << ln.in_off_bytes
            << ",\"out_off\":" << ln.out_off_bytes << ",\"tw4_n\":" << d.tw4_n << ",\"radix\":[";
@@ -976,8 +996,9 @@ static hipError_t launch_all(Plan* p, Slot* s, const void* in, void* out, hipStr
         d.in = role_ptr(ln.in_role) + ln.in_off_bytes;
         d.out = role_ptr(ln.out_role) + ln.out_off_bytes;
         void* args[] = {&d};
-        hipError_t e = hipLaunchKernel(kernel_symbol(ln.kind, p->spec.prec_bytes, ln.all_radix, ln.E, ln.maxt),
-                                       dim3(ln.grid),
+        const void* fn = ln.fixed ? kernel_symbol_fixed(p->spec.prec_bytes, ln.d.L)
+                                  : kernel_symbol(ln.kind, p->spec.prec_bytes, ln.all_radix, ln.E, ln.maxt);
+        hipError_t e = hipLaunchKernel(fn, dim3(ln.grid),
                                        dim3(ln.block), args, ln.lds_bytes, stream);
         if (e != hipSuccess) return e;
         if (p->profile) {

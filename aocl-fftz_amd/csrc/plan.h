@@ -40,6 +40,7 @@ struct Launch {
     bool all_radix = false;
     int E = 16;        // complex values per thread per stage
     int maxt = 256;    // block-size bound of the kernel variant
+    bool fixed = false;  // fixed-length kernel (kernels_fixed.hip)
     PassDesc d{};
     Role in_role = Role::In, out_role = Role::Out;
     int64_t in_off_bytes = 0, out_off_bytes = 0;

@@ -68,7 +68,7 @@ EXPORTS = (
     "aoclfftz_gpu_set_stream", "aoclfftz_gpu_synchronize", "aoclfftz_gpu_set_graph",
     "aoclfftz_gpu_plan_info", "aoclfftz_gpu_plan_flops", "aoclfftz_gpu_plan_device_bytes",
     "aoclfftz_gpu_plan_json", "aoclfftz_gpu_profile", "aoclfftz_gpu_profile_read",
-    "aoclfftz_gpu_plan_launches",
+    "aoclfftz_gpu_plan_launches", "aoclfftz_gpu_execute_many",
 )
 
 _lib = None
@@ -107,6 +107,8 @@ def load():
     lib.aoclfftz_gpu_plan_flops.restype = ctypes.c_double
     lib.aoclfftz_gpu_plan_json.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_char_p, ctypes.c_int64]
     lib.aoclfftz_gpu_plan_json.restype = ctypes.c_int64
+    lib.aoclfftz_gpu_execute_many.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.c_int32]
+    lib.aoclfftz_gpu_execute_many.restype = ctypes.c_int
     lib.aoclfftz_gpu_profile.argtypes = [vp, ctypes.c_int32]
     lib.aoclfftz_gpu_profile.restype = ctypes.c_int
     lib.aoclfftz_gpu_profile_read.argtypes = [vp, ctypes.POINTER(ctypes.c_float), ctypes.c_int32]
@@ -185,6 +187,13 @@ class Plan:
 
     def execute_io(self, inp, out) -> int:
         return load().aoclfftz_execute_io(self.handle, _ptr(inp), _ptr(out))
+
+    def execute_many(self, ins, outs) -> int:
+        """Execute on each (ins[i], outs[i]) pair, in order, in one call."""
+        n = len(ins)
+        a = (ctypes.c_void_p * n)(*[_ptr(x) for x in ins])
+        b = (ctypes.c_void_p * n)(*[_ptr(x) for x in outs])
+        return load().aoclfftz_gpu_execute_many(self.handle, a, b, n)
 
     def set_stream(self, stream) -> int:
         s = 0 if stream is None else (stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream))

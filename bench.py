@@ -38,6 +38,16 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def _traffic(kernel):
+    """HBM bytes per dispatch of `kernel` measured with rocprofv3 FETCH_SIZE /
+    WRITE_SIZE (calibrated on a plain copy), from the committed summary."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            return json.load(f).get(kernel)
+    except OSError:
+        return None
+
+
 def launch_bytes(plan_json):
     """Algorithmic HBM bytes of each launch: every element read once and
     written once."""
@@ -146,25 +156,28 @@ def bench_batched_n1024(torch, fftz, steps, warmup, n=1024, b=4096):
     pj = fftz.plan_json([(n, 1, 1)], [(b, n, n)], "f", 0)
     bpl = launch_bytes(pj)
     log(f"[bench] n1024 fp32 plan: {p.info}")
-    for i in range(warmup * nbuf):
-        p.execute_io(xs[i % nbuf], ys[i % nbuf])
+    p.execute_many([xs[i % nbuf] for i in range(warmup * nbuf)], [ys[i % nbuf] for i in range(warmup * nbuf)])
     torch.cuda.synchronize()
     reps = max(steps, 1) * 20
+    ins = [xs[i % nbuf] for i in range(reps)]
+    outs = [ys[i % nbuf] for i in range(reps)]
     t0 = torch.cuda.Event(enable_timing=True)
     t1 = torch.cuda.Event(enable_timing=True)
-    p.profile(True)
+    # one C call enqueues all executes, so the host never starves the GPU
     t0.record(stream)
-    for i in range(reps):
-        p.execute_io(xs[i % nbuf], ys[i % nbuf])
+    assert p.execute_many(ins, outs) == 0
     t1.record(stream)
     torch.cuda.synchronize()
     ms_cold = t0.elapsed_time(t1) / reps
+    # kernel durations with events around each launch (separate pass)
+    p.profile(True)
+    assert p.execute_many(ins, outs) == 0
+    torch.cuda.synchronize()
     prof = p.profile_read()
     p.profile(False)
     # cache-warm: the same buffers every time
     t0.record(stream)
-    for i in range(reps):
-        p.execute_io(xs[0], ys[0])
+    p.execute_many([xs[0]] * reps, [ys[0]] * reps)
     t1.record(stream)
     torch.cuda.synchronize()
     ms_warm = t0.elapsed_time(t1) / reps
@@ -190,6 +203,28 @@ def bench_batched_n1024(torch, fftz, steps, warmup, n=1024, b=4096):
     del xs, ys
     torch.cuda.empty_cache()
     return res
+
+
+def copy_ceiling(torch, mib=256, reps=20):
+    """Plain device-to-device copy of a known size: the practical HBM
+    ceiling on this box, and the calibration kernel for FETCH/WRITE_SIZE."""
+    n = mib * (1 << 20) // 4
+    a = torch.empty(n, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    t0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    t1.record()
+    torch.cuda.synchronize()
+    ms = t0.elapsed_time(t1) / reps
+    del a, b
+    torch.cuda.empty_cache()
+    return 2 * mib * (1 << 20) / (ms * 1e-3)
 
 
 def cpu_baseline(res_2p24, res_1024, budget_s):
@@ -252,6 +287,8 @@ def main():
     import fftz
     assert torch.cuda.is_available()
     fftz.load()
+    # time on a dedicated (default-priority) stream, not the legacy NULL stream
+    torch.cuda.set_stream(torch.cuda.Stream())
     log("[bench] start 2^24 fp64 round trip")
     r24 = bench_roundtrip_2p24(torch, fftz, args.steps, args.warmup)
     log(f"[bench] 2^24: {r24['ms_per_step']:.3f} ms/step {r24['gflops']:.1f} GFLOP/s")
@@ -259,6 +296,8 @@ def main():
     r10 = bench_batched_n1024(torch, fftz, args.steps, args.warmup)
     log(f"[bench] n1024: {r10['ms_per_execute'] * 1e3:.2f} us {r10['gflops']:.1f} GFLOP/s "
         f"frac {r10['frac_of_hbm']:.3f}")
+    copy_bps = copy_ceiling(torch)
+    log(f"[bench] copy ceiling {copy_bps / 1e9:.0f} GB/s")
     base, extra = (None, {})
     if not args.no_cpu:
         log("[bench] cpu baseline (reference, bounded sample)")
@@ -281,7 +320,7 @@ def main():
                                "(four-step path)", "n": 1 << 24, "batch": 1, "transforms_per_step": 2},
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4),
-                     "traffic": None,
+                     "traffic": _traffic("k_fixed_pass<double, 256>"),
                      "kernel": f"k_fft_pass launch #{r24['dominant_launch']} of the four-step plan",
                      "algorithmic_bytes_per_launch": r24["launch_bytes"][r24["dominant_launch"]]},
         "cpu_baseline": base,
@@ -291,6 +330,7 @@ def main():
         "batched_n1024_fp32": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r10.items()
                                if not k.startswith("_")},
     }
+    line["copy_ceiling_GBps"] = round(copy_bps / 1e9, 1)
     line.update(extra)
     print(json.dumps(line), flush=True)
 

@@ -15,12 +15,20 @@ extern "C" {
 #endif
 
 /* Bind a HIP stream (hipStream_t passed as void*) to the handle.  Afterwards
- * aoclfftz_execute / aoclfftz_execute_io on DEVICE buffers enqueue on that
- * stream and return without waiting; host buffers still run synchronously.
- * Passing NULL restores the default synchronous behaviour.  Calls that share
+ * aoclfftz_execute / aoclfftz_execute_io enqueue on that stream and return
+ * without waiting.  In this mode in / out MUST be device memory (hipMalloc,
+ * managed, or torch device tensors): the call does not inspect the pointers.
+ * Passing NULL restores the default synchronous behaviour, where host
+ * buffers are also accepted and staged.  Calls that share
  * one handle in stream mode are ordered by the stream; do not use one handle
  * from several streams at once. */
 aoclfftz_error_type aoclfftz_gpu_set_stream(FFTZ_VOID *handle, FFTZ_VOID *stream);
+
+/* Execute the plan on count (in, out) buffer pairs in one call, in order, with
+ * the semantics of aoclfftz_execute_io for each pair (asynchronous in stream
+ * mode).  Saves the per-call overhead of foreign-function callers. */
+aoclfftz_error_type aoclfftz_gpu_execute_many(FFTZ_VOID *handle, FFTZ_VOID **ins, FFTZ_VOID **outs,
+                                             FFTZ_INT32 count);
 
 /* Wait for all work the handle has enqueued. */
 aoclfftz_error_type aoclfftz_gpu_synchronize(FFTZ_VOID *handle);

@@ -133,5 +133,16 @@ def check_launch_limits(plan, lds_cap=163840):
         need = L + (1 if ln["flags"] & K_C2R_PRE else 0)
         assert ln["LP"] >= (need - 1) + ((need - 1) >> 4) + 1
         assert ln["lds"] == C * ln["LP"] * (16 if plan["prec"] == 8 else 8)
-        assert ln["grid"] * C >= ln["nlines"]
+        ntiles = -(-ln["nlines"] // C)
+        G = ln["grid"]
+        # one tile per workgroup; the XCD remap must be a permutation
+        assert G == ntiles
+        tiles = []
+        per = G >> 3
+        for b in range(G):
+            t0 = b
+            if ln["flags"] & 64 and b < per * 8:
+                t0 = (b & 7) * per + (b >> 3)
+            tiles.append(t0)
+        assert sorted(tiles) == list(range(ntiles)), "tile remap not a permutation"
         assert ln["nlines"] < 2 ** 31
