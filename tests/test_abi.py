@@ -27,6 +27,13 @@ def declared_functions():
         for m in re.finditer(r"\b((?:aoclfftz|fftwf?)_\w+)\s*\(", src):
             if not m.group(1).endswith("_t"):
                 names.add(m.group(1))
+        # FFTW declarations are generated per precision: X##name(...)
+        if "FFTZ_FFTW_DECLARE" in src:
+            body = src[src.index("#define FFTZ_FFTW_DECLARE"):src.index("FFTZ_FFTW_DECLARE(fftw_")]
+            for m in re.finditer(r"X##(\w+)\s*(?:\(|\[)", body):
+                if m.group(1) not in ("plan", "iodim", "iodim64", "r2r_kind", "write_char_func", "read_char_func"):
+                    names.add("fftw_" + m.group(1))
+                    names.add("fftwf_" + m.group(1))
     return sorted(names)
 
 
@@ -36,6 +43,7 @@ def test_library_exports_every_declared_symbol():
     assert "aoclfftz_setup_f" in names and "aoclfftz_execute_io" in names
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
+    assert "fftw_plan_many_dft" in names and "fftwf_execute_dft" in names and "fftw_version" in names
     assert set(fftz.EXPORTS) <= set(names)
 
 
