@@ -457,6 +457,7 @@ const FFTZ_CHAR* aoclfftz_gpu_plan_info(FFTZ_VOID* handle)
            << " lds=" << ln.lds_bytes << ((ln.d.flags & fftz::kInCol) ? " incol" : "")
            << ((ln.d.flags & fftz::kOutCol) ? " outcol" : "") << ((ln.d.flags & fftz::kTw4) ? " tw4" : "") << "]";
     }
+    if (p->is_batch) os << " " << p->description;
     os << " work=" << p->work_bytes;
     h->info = os.str();
     return h->info.c_str();
@@ -523,7 +524,66 @@ FFTZ_INT32 aoclfftz_gpu_profile_read(FFTZ_VOID* handle, float* ms, FFTZ_INT32 ca
 FFTZ_INT32 aoclfftz_gpu_plan_launches(FFTZ_VOID* handle)
 {
     Handle* h = as_handle(handle);
-    return h ? (FFTZ_INT32)h->plan->launches.size() : -1;
+    if (!h) return -1;
+    return (FFTZ_INT32)(h->plan->is_batch ? h->plan->bclasses.size() : h->plan->launches.size());
+}
+
+static bool batch_items(const aoclfftz_gpu_batch_item* items, FFTZ_INT64 count, std::vector<fftz::BatchItem>& v)
+{
+    if (!items || count < 0) return false;
+    v.resize((size_t)count);
+    for (FFTZ_INT64 i = 0; i < count; i++)
+        v[(size_t)i] = {items[i].n, items[i].batch, items[i].in_offset, items[i].out_offset, items[i].dist};
+    return true;
+}
+
+FFTZ_VOID* aoclfftz_gpu_batch_setup(const aoclfftz_gpu_batch_item* items, FFTZ_INT64 count,
+                                    FFTZ_INT32 precision_bytes, FFTZ_INT32 direction)
+{
+    const char* F = "aoclfftz_gpu_batch_setup";
+    std::vector<fftz::BatchItem> v;
+    if (!batch_items(items, count, v) || (precision_bytes != 4 && precision_bytes != 8) ||
+        (direction != 0 && direction != 1)) {
+        report_error(F, "invalid arguments");
+        return nullptr;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        (void)hipGetLastError();
+        report_error(F, "no HIP device available");
+        return nullptr;
+    }
+    std::string err;
+    fftz::Plan* plan = fftz::batch_plan_create(v, precision_bytes, direction, err);
+    if (!plan) {
+        report_error(F, "plan creation failed: %s", err.c_str());
+        return nullptr;
+    }
+    Handle* h = new Handle();
+    h->plan = plan;
+    return h;
+}
+
+FFTZ_INT64 aoclfftz_gpu_batch_plan_json(const aoclfftz_gpu_batch_item* items, FFTZ_INT64 count,
+                                        FFTZ_INT32 precision_bytes, FFTZ_INT32 direction, FFTZ_CHAR* buf,
+                                        FFTZ_INT64 buflen)
+{
+    std::vector<fftz::BatchItem> v;
+    if (!batch_items(items, count, v) || (precision_bytes != 4 && precision_bytes != 8)) return -1;
+    std::string err;
+    fftz::Plan* plan = fftz::batch_plan_create(v, precision_bytes, direction, err, true);
+    if (!plan) {
+        report_error("aoclfftz_gpu_batch_plan_json", "plan creation failed: %s", err.c_str());
+        return -1;
+    }
+    std::string js = fftz::plan_json(plan);
+    fftz::plan_destroy(plan);
+    if (buf && buflen > 0) {
+        const size_t n = std::min((size_t)(buflen - 1), js.size());
+        memcpy(buf, js.data(), n);
+        buf[n] = 0;
+    }
+    return (FFTZ_INT64)js.size();
 }
 
 FFTZ_DOUBLE aoclfftz_gpu_plan_flops(FFTZ_VOID* handle)

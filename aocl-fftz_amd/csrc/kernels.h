@@ -1,6 +1,8 @@
 // Kernel symbols exported from kernels.hip to the host planner.
 #pragma once
 
+#include "pass.h"
+
 namespace fftz {
 
 enum class KernelKind { Pass, Bluestein };
@@ -12,6 +14,11 @@ enum class KernelKind { Pass, Bluestein };
 // thread per stage (8 or 16).  maxt: block-size bound of the variant (256 or
 // 1024); the 256 variants get a larger VGPR budget.
 const void* kernel_symbol(KernelKind kind, int prec_bytes, bool all_radix, int E, int maxt);
+
+// Uneven-batch persistent kernel (kernels.hip::k_batch) and the counter
+// reset kernel launched before it.
+const void* kernel_symbol_batch(int prec_bytes, int E, int maxt);
+const void* kernel_symbol_reset();
 
 // Fixed-length pass kernel for power-of-two L in [64, 4096] (radix plan 16s
 // then remainder, 16 elements per thread, block <= 256); nullptr otherwise.

@@ -59,15 +59,6 @@ __device__ __forceinline__ cplx<double> tw4_root(const PassDesc& p, int64_t q)
     return cmul(hi, lo);
 }
 
-template <typename R>
-__device__ __forceinline__ cplx<R> apply_tw4(const PassDesc& p, cplx<R> v, uint32_t l0, int e)
-{
-    cplx<double> w = tw4_root(p, (int64_t)l0 * e);
-    cplx<double> vd = {double(v.x), double(v.y)};
-    cplx<double> r = cmul(vd, w);
-    return {R(r.x), R(r.y)};
-}
-
 // ---------------------------------------------------------------------------
 // Tile load / store.
 // Row passes: thread -> (line c = tid / TL, lane t = tid % TL): a wavefront
@@ -133,6 +124,7 @@ __device__ __forceinline__ void c2r_separate(const PassDesc& p, cplx<R>* lds, in
 {
     const int c = tid >> p.tl_shift;
     const int t = tid & (p.TL - 1);
+    if (c >= p.C) return;
     cplx<R>* line = lds + c * p.LP;
     const cplx<R>* rtw = reinterpret_cast<const cplx<R>*>(p.rtw);
     const int L = p.L;
@@ -168,11 +160,12 @@ __device__ __forceinline__ void stage(const PassDesc& p, cplx<R>* lds, int tid, 
     const int nbl = p.L / RAD;
     const int ls = p.ls[s];
     cplx<R>* line = lds + c * p.LP;
+    const bool act = c < p.C;  // batch kernel: threads beyond C * TL idle
     cplx<R> v[K][RAD];
 #pragma unroll
     for (int k = 0; k < K; k++) {
         const int j = t + k * p.TL;
-        if (j < nbl) {
+        if (act && j < nbl) {
 #pragma unroll
             for (int i = 0; i < RAD; i++) v[k][i] = line[lds_pad(i * nbl + j)];
         }
@@ -182,7 +175,7 @@ __device__ __forceinline__ void stage(const PassDesc& p, cplx<R>* lds, int tid, 
 #pragma unroll
     for (int k = 0; k < K; k++) {
         const int j = t + k * p.TL;
-        if (j < nbl) {
+        if (act && j < nbl) {
             const int g = (int)fdiv((uint32_t)j, p.lsdiv[s]);
             const int cc = j - g * ls;
             if (ls > 1) {
@@ -285,10 +278,19 @@ __device__ __forceinline__ void store_tile(const PassDesc& p, const cplx<R>* lds
         if (e < p.nvalid_out) v[k] = line[lds_pad(e)];
     }
     if (p.flags & kTw4) {
+        // w^(l0 e) for e = t + k TL: two table products, then an FP64
+        // recurrence over k
+        cplx<double> wc = tw4_root(p, (int64_t)li.l0 * t);
+        const cplx<double> ws = tw4_root(p, (int64_t)li.l0 * p.TL);
 #pragma unroll
         for (int k = 0; k < E; k++) {
             const int e = t + k * p.TL;
-            if (e < p.nvalid_out) v[k] = apply_tw4(p, v[k], li.l0, e);
+            if (e < p.nvalid_out) {
+                cplx<double> vd = {double(v[k].x), double(v[k].y)};
+                vd = cmul(vd, wc);
+                v[k] = {R(vd.x), R(vd.y)};
+            }
+            wc = cmul(wc, ws);
         }
     }
     if (p.flags & kScale) {
@@ -336,13 +338,9 @@ __device__ __forceinline__ uint32_t tile_of_block(const PassDesc& p)
     return tile;
 }
 
-template <typename R, bool ALLRAD, int E, int MAXT>
-__global__ void __launch_bounds__(MAXT) k_fft_pass(const PassDesc p)
+template <typename R, bool ALLRAD, int E>
+__device__ __forceinline__ void pass_body(const PassDesc& p, cplx<R>* lds, int64_t l_base, int tid)
 {
-    extern __shared__ unsigned char smem_raw[];
-    cplx<R>* lds = reinterpret_cast<cplx<R>*>(smem_raw);
-    const int tid = threadIdx.x;
-    const int64_t l_base = (int64_t)tile_of_block(p) * p.C;
     load_tile<R, E>(p, lds, l_base, tid);
     __syncthreads();
     if (p.flags & kC2RPre) {
@@ -357,6 +355,16 @@ __global__ void __launch_bounds__(MAXT) k_fft_pass(const PassDesc p)
     }
 }
 
+template <typename R, bool ALLRAD, int E, int MAXT>
+__global__ void __launch_bounds__(MAXT) k_fft_pass(const PassDesc p)
+{
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+    cplx<R>* lds = reinterpret_cast<cplx<R>*>(smem_raw);
+    const int tid = threadIdx.x;
+    const int64_t l_base = (int64_t)tile_of_block(p) * p.C;
+    pass_body<R, ALLRAD, E>(p, lds, l_base, tid);
+}
+
 // ---------------------------------------------------------------------------
 // Bluestein (chirp-z) for one or more lines of prime length n, fused into a
 // single pass over an M-point LDS tile (M = smooth length >= 2n-1, the
@@ -369,13 +377,9 @@ __global__ void __launch_bounds__(MAXT) k_fft_pass(const PassDesc p)
 // which is core/solvers/complex/bluestein_solver_dft.c:137-234 with its five
 // sweeps (pre_mul, memset, FFT, mul, IFFT, post_mul) in one HBM round trip.
 // ---------------------------------------------------------------------------
-template <typename R, bool ALLRAD, int E, int MAXT>
-__global__ void __launch_bounds__(MAXT) k_bluestein(const PassDesc p)
+template <typename R, bool ALLRAD, int E>
+__device__ __forceinline__ void bluestein_body(const PassDesc& p, cplx<R>* lds, int64_t l_base, int tid)
 {
-    extern __shared__ unsigned char smem_raw[];
-    cplx<R>* lds = reinterpret_cast<cplx<R>*>(smem_raw);
-    const int tid = threadIdx.x;
-    const int64_t l_base = (int64_t)blockIdx.x * p.C;
     const int c = tid >> p.tl_shift;
     const int t = tid & (p.TL - 1);
     const int64_t l = l_base + c;
@@ -401,9 +405,11 @@ __global__ void __launch_bounds__(MAXT) k_bluestein(const PassDesc p)
     }
     __syncthreads();
     run_stages<R, ALLRAD, E>(p, lds, tid);
-    for (int e = t; e < M; e += p.TL) {
-        cplx<R> a = line[lds_pad(e)];
-        line[lds_pad(e)] = cmul(cconj(a), spec[e]);
+    if (c < p.C) {
+        for (int e = t; e < M; e += p.TL) {
+            cplx<R> a = line[lds_pad(e)];
+            line[lds_pad(e)] = cmul(cconj(a), spec[e]);
+        }
     }
     __syncthreads();
     run_stages<R, ALLRAD, E>(p, lds, tid);
@@ -417,6 +423,72 @@ __global__ void __launch_bounds__(MAXT) k_bluestein(const PassDesc p)
             dst[(int64_t)k * p.oes] = y;
         }
     }
+}
+
+template <typename R, bool ALLRAD, int E, int MAXT>
+__global__ void __launch_bounds__(MAXT) k_bluestein(const PassDesc p)
+{
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+    cplx<R>* lds = reinterpret_cast<cplx<R>*>(smem_raw);
+    bluestein_body<R, ALLRAD, E>(p, lds, (int64_t)blockIdx.x * p.C, threadIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// Uneven batches in one persistent launch.  A device work queue lists tiles
+// (size class, first line, line count, offsets); each workgroup takes the
+// next tile from an atomic counter, copies the size class's pass descriptor
+// into LDS, patches the tile's buffers in, and runs the same pass or
+// Bluestein body as the single-size kernels.  Tiles are sorted by cost on
+// the host, largest first.  The arithmetic of a tile does not depend on the
+// workgroup that runs it, so results are bitwise identical run to run.
+// ---------------------------------------------------------------------------
+template <typename R, int E, int MAXT>
+__global__ void __launch_bounds__(MAXT) k_batch(const BatchArgs a)
+{
+    // one dynamic LDS region, laid out explicitly: [descriptor | tile index |
+    // tile data], each part 16-byte aligned (static __shared__ variables next
+    // to a dynamic array would leave the array misaligned for 8/16-byte LDS
+    // accesses)
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+    PassDesc& pd = *reinterpret_cast<PassDesc*>(smem_raw);
+    int& s_tile = *reinterpret_cast<int*>(smem_raw + kBatchHdrBytes - 16);
+    cplx<R>* lds = reinterpret_cast<cplx<R>*>(smem_raw + kBatchHdrBytes);
+    const int tid = threadIdx.x;
+    constexpr int kWords = sizeof(PassDesc) / 4;
+    for (;;) {
+        if (tid == 0) s_tile = (int)atomicAdd(a.counter, 1u);
+        __syncthreads();
+        const int t = s_tile;
+        if (t >= a.ntiles) break;
+        const BatchTile bt = a.tiles[t];
+        // word-parallel copy through memcpy (char-typed accesses may alias
+        // the descriptor's pointer / int64 / double fields; plain uint32_t
+        // stores would let the compiler keep the previous tile's values)
+        const char* src = reinterpret_cast<const char*>(a.descs + bt.desc);
+        char* dstb = reinterpret_cast<char*>(&pd);
+        for (int w = tid; w < kWords; w += MAXT) __builtin_memcpy(dstb + 4 * w, src + 4 * w, 4);
+        __syncthreads();
+        if (tid == 0) {
+            const int esz = 2 * (int)sizeof(R);
+            pd.in = reinterpret_cast<const char*>(a.in) + bt.in_off * esz;
+            pd.out = reinterpret_cast<char*>(a.out) + bt.out_off * esz;
+            pd.nlines = bt.nlines;
+            pd.ils[0] = bt.dist;
+            pd.ols[0] = bt.dist;
+        }
+        __syncthreads();
+        if (bt.bluestein) {
+            bluestein_body<R, true, E>(pd, lds, 0, tid);
+        } else {
+            pass_body<R, true, E>(pd, lds, 0, tid);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_reset_counters(unsigned* c, int n)
+{
+    if ((int)threadIdx.x < n) c[threadIdx.x] = 0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -434,6 +506,18 @@ static const void* sym_et(KernelKind kind, int E, int maxt)
     if (E == 8) return maxt <= 256 ? sym<R, A, 8, 256>(kind) : sym<R, A, 8, 1024>(kind);
     return maxt <= 256 ? sym<R, A, 16, 256>(kind) : sym<R, A, 16, 1024>(kind);
 }
+
+const void* kernel_symbol_batch(int prec_bytes, int E, int maxt)
+{
+    if (prec_bytes == 4) {
+        if (E == 8) return maxt <= 256 ? (const void*)&k_batch<float, 8, 256> : (const void*)&k_batch<float, 8, 1024>;
+        return maxt <= 256 ? (const void*)&k_batch<float, 16, 256> : (const void*)&k_batch<float, 16, 1024>;
+    }
+    if (E == 8) return maxt <= 256 ? (const void*)&k_batch<double, 8, 256> : (const void*)&k_batch<double, 8, 1024>;
+    return maxt <= 256 ? (const void*)&k_batch<double, 16, 256> : (const void*)&k_batch<double, 16, 1024>;
+}
+
+const void* kernel_symbol_reset() { return (const void*)&k_reset_counters; }
 
 const void* kernel_symbol(KernelKind kind, int prec_bytes, bool all_radix, int E, int maxt)
 {

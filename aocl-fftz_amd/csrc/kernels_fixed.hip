@@ -150,7 +150,7 @@ __global__ void __launch_bounds__(256) k_fixed_pass(const PassDesc p)
     constexpr int R0 = P::radix(0);
     constexpr int RL = P::radix(S - 1);
     static_assert(R0 == 16, "first stage is radix 16");
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     cplx<R>* lds = reinterpret_cast<cplx<R>*>(smem_raw);
     const int tid = threadIdx.x;
     uint32_t tile = blockIdx.x;
@@ -204,23 +204,31 @@ __global__ void __launch_bounds__(256) k_fixed_pass(const PassDesc p)
     cplx<R>* dst = reinterpret_cast<cplx<R>*>(p.out) + lo.ooff;
     const cplx<double>* t4 = reinterpret_cast<const cplx<double>*>(p.tw4);
     const int64_t lo_n = (int64_t)1 << p.tw4_lobits;
+    auto root = [&](int64_t q) -> cplx<double> {
+        return cmul(t4[lo_n + (q >> p.tw4_lobits)], t4[q & (lo_n - 1)]);
+    };
+    const bool tw4 = (p.flags & kTw4) != 0;
+    // four-step twiddle w_N^(l0 e) for e = i*lsL + j: one table product for
+    // w^(l0 j) and one for the step w^(l0 lsL), then an FP64 recurrence over
+    // i (at most RL-1 = 15 products, ~2e-15 relative)
+    cplx<double> wstep = {1.0, 0.0};
+    if (tw4) wstep = root((int64_t)lo.l0 * lsL);
 #pragma unroll
     for (int k = 0; k < KL; k++) {
         const int j = to + k * TL;  // cc = j, g = 0
         twiddle_legs<R, RL>(w[k], twl + j * (RL - 1));
         Dft<RL, R>::run(w[k]);
+        cplx<double> wc = {1.0, 0.0};
+        if (tw4) wc = root((int64_t)lo.l0 * j);
 #pragma unroll
         for (int i = 0; i < RL; i++) {
             const int e = i * lsL + j;
             cplx<R> y = w[k][i];
-            if (p.flags & kTw4) {
-                const int64_t q = (int64_t)lo.l0 * e;
-                cplx<double> a = t4[q & (lo_n - 1)];
-                cplx<double> b = t4[lo_n + (q >> p.tw4_lobits)];
-                cplx<double> ww = cmul(b, a);
+            if (tw4) {
                 cplx<double> yd = {double(y.x), double(y.y)};
-                yd = cmul(yd, ww);
+                yd = cmul(yd, wc);
                 y = {R(yd.x), R(yd.y)};
+                wc = cmul(wc, wstep);
             }
             if (p.flags & kScale) y = cscale(y, R(sizeof(R) == 4 ? p.scale_f : p.scale_d));
             if (p.flags & kConjOut) y.y = -y.y;

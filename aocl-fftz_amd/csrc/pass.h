@@ -89,6 +89,30 @@ struct PassDesc {
     int twoff[kMaxStages];
 };
 
+// One tile of an uneven batch: `nlines` consecutive transforms of size class
+// `desc`, `dist` elements apart, starting at in_off / out_off (elements).
+struct BatchTile {
+    uint32_t desc;
+    uint32_t nlines;
+    uint32_t bluestein;
+    uint32_t pad_;
+    int64_t in_off, out_off;
+    int64_t dist;
+};
+
+// Bytes at the start of the batch kernel's dynamic LDS that hold the tile's
+// pass descriptor and the tile index (rounded to 256).
+constexpr int kBatchHdrBytes = (int)(((sizeof(PassDesc) + 16 + 255) / 256) * 256);
+
+struct BatchArgs {
+    const PassDesc* descs;
+    const BatchTile* tiles;
+    int ntiles;
+    unsigned* counter;
+    const void* in;
+    void* out;
+};
+
 // LDS index of element e inside one line: one pad slot every 16 elements so
 // the stride-R writes of the first Stockham stages spread over the banks.
 __host__ __device__ inline int lds_pad(int e) { return e + (e >> 4); }

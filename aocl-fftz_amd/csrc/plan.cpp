@@ -576,6 +576,21 @@ static bool plan_axis_c2c(Plan* plan, TableBuilder& tb, const std::vector<AxisSp
 // and a product of radices.
 static bool split_factors(int64_t N, int64_t fmax, std::vector<int64_t>& f)
 {
+    if (const char* forced = getenv("FFTZ_FACTORS")) {
+        // explicit factor list (tuning experiments): "a,b[,c]" with product N
+        std::vector<int64_t> g;
+        int64_t prod = 1;
+        for (const char* c = forced; *c;) {
+            int64_t v = strtoll(c, (char**)&c, 10);
+            if (v > 1) { g.push_back(v); prod *= v; }
+            if (*c == ',') c++;
+            else if (*c) break;
+        }
+        if (prod == N && g.size() >= 2 && g.size() <= 3) {
+            f = g;
+            return true;
+        }
+    }
     for (int p = 2; p <= 3; p++) {
         // candidate first factors near N^(1/p)
         const double target = std::pow((double)N, 1.0 / p);
@@ -615,12 +630,22 @@ static bool plan_large_1d(Plan* plan, TableBuilder& tb, const AxisSpec& dim, con
 {
     const int prec = plan->spec.prec_bytes;
     const int64_t N = dim.n;
-    const int64_t fmax = prec == 8 ? env_int("FFTZ_FMAX_D", 512) : env_int("FFTZ_FMAX_F", 1024);
+    // Largest pass length: 256 keeps column tiles 16 lines wide (256-byte
+    // FP64 / 128-byte FP32 segments) with 256-thread workgroups; grow it only
+    // when three passes of 256 cannot cover N.
+    int64_t fmax = env_int(prec == 8 ? "FFTZ_FMAX_D" : "FFTZ_FMAX_F", 256);
+    while (fmax < 4096 && fmax * fmax * fmax < N) fmax *= 2;
     std::vector<int64_t> f;
     if (!split_factors(N, fmax, f)) {
         err = "no four-step factorisation for length " + std::to_string(N);
         return false;
     }
+    // power-of-two factors last: the final (transposing) pass then runs on
+    // the fixed-length kernel
+    if (!getenv("FFTZ_FACTORS"))
+        std::stable_sort(f.begin(), f.end(), [](int64_t a, int64_t b) {
+            return ((a & (a - 1)) != 0) > ((b & (b - 1)) != 0);
+        });
     std::vector<AxisSpec> bv = vecs;
     if (bv.size() > 1) {
         err = "large 1-D transform with more than one batch axis";
@@ -802,6 +827,220 @@ static bool plan_real(Plan* plan, TableBuilder& tb, std::vector<AxisSpec> dims, 
     return true;
 }
 
+// ---------------------------------------------------------------------------
+// Uneven batches (many lengths, one persistent launch per tile class).
+// The reference runs such a list as independent plans, one setup/execute per
+// (n, batch) item; here every distinct length becomes a pass descriptor in the
+// plan arena and every item a run of tiles in a device work queue.
+// ---------------------------------------------------------------------------
+namespace {
+struct SizeInfo {
+    int64_t n = 0;
+    bool bluestein = false;
+    int L = 0;
+    int cls = 0;
+    int C = 1;
+    PassDesc d{};
+};
+}  // namespace
+
+static bool batch_size_desc(TableBuilder& tb, int prec, int64_t n, int E, int maxt, size_t lds_budget,
+                            SizeInfo& si, std::string& err)
+{
+    si.n = n;
+    si.bluestein = !smooth13(n);
+    const int64_t L = si.bluestein ? bluestein_length(n) : n;
+    si.L = (int)L;
+    PassDesc& d = si.d;
+    d = PassDesc{};
+    std::vector<int> radix;
+    if (!factor_radices((int)L, radix)) {
+        err = "no radix plan for batch length " + std::to_string(L);
+        return false;
+    }
+    d.L = (int)L;
+    d.nst = (int)radix.size();
+    for (size_t s = 0; s < radix.size(); s++) d.radix[s] = radix[s];
+    const int64_t stw = stage_twiddles(tb, prec, radix, d.twoff, d.ls);
+    for (int s = 0; s < d.nst; s++) d.lsdiv[s] = make_fastdiv((uint32_t)d.ls[s]);
+    int TL = next_pow2((L + E - 1) / E);
+    if (TL < 1) TL = 1;
+    if (TL > maxt) {
+        err = "batch length " + std::to_string(L) + " too long for its tile class";
+        return false;
+    }
+    d.TL = TL;
+    d.tl_shift = ilog2(TL);
+    int LP = lds_pad((int)L - 1) + 1;
+    if ((LP & 1) == 0) LP++;
+    d.LP = LP;
+    const size_t line_bytes = (size_t)LP * 2 * prec;
+    if (line_bytes > lds_budget) {
+        err = "batch length " + std::to_string(L) + " exceeds LDS";
+        return false;
+    }
+    int C = 1;
+    const int cmax = env_int("FFTZ_BATCH_CMAX", 64);
+    while (C * 2 * TL <= maxt && (size_t)(C * 2) * line_bytes <= lds_budget && C * 2 <= cmax) C *= 2;
+    si.C = C;
+    d.C = C;
+    d.c_shift = ilog2(C);
+    d.ies = 1;
+    d.oes = 1;
+    d.ln0 = make_fastdiv(1u << 30);  // l0 = line index (< 2^30)
+    d.ln1 = make_fastdiv(1);
+    d.ils[1] = d.ils[2] = d.ols[1] = d.ols[2] = 0;
+    d.nvalid_in = si.bluestein ? (int)n : (int)L;
+    d.nvalid_out = si.bluestein ? (int)n : (int)L;
+    d.scale_f = 1.0f;
+    d.scale_d = 1.0;
+    // table pointers hold arena offsets until upload (resolved in place)
+    d.stw = (const void*)(intptr_t)stw;
+    if (si.bluestein) {
+        int64_t co, so;
+        bluestein_tables(tb, prec, n, L, &co, &so);
+        d.chirp = (const void*)(intptr_t)co;
+        d.spec = (const void*)(intptr_t)so;
+    }
+    return true;
+}
+
+Plan* batch_plan_create(const std::vector<BatchItem>& items, int prec, int dir, std::string& err, bool dry_run)
+{
+    Plan* plan = new Plan();
+    plan->spec.prec_bytes = prec;
+    plan->spec.dir = dir;
+    plan->spec.type = FftType::C2C;
+    plan->is_batch = true;
+    if (!dry_run) {
+        (void)hipGetDevice(&plan->device);
+        (void)device_info();
+    }
+    const DeviceInfo& dev = device_info();
+    const int esize = 2 * prec;
+    TableBuilder tb{plan->host_tables};
+    // size classes: small tiles (E = 8, 256 threads, <= 64 KiB LDS, several
+    // workgroups per CU) and large tiles (E = 16, 1024 threads, whole LDS)
+    const size_t static_lds = kBatchHdrBytes;
+    const size_t small_budget = (size_t)env_int("FFTZ_BATCH_SMALL_LDS", 64 * 1024);
+    const size_t big_budget = dev.lds_per_block - static_lds;
+    std::map<int64_t, int> size_index;
+    std::vector<SizeInfo> sizes;
+    int64_t in_ext = 0, out_ext = 0;
+    double flops = 0.0;
+    for (const BatchItem& it : items) {
+        if (it.n <= 0 || it.batch < 0 || it.in_off < 0 || it.out_off < 0 || it.dist < 0) {
+            err = "invalid batch item";
+            delete plan;
+            return nullptr;
+        }
+        if (it.batch == 0) continue;
+        const int64_t dist = it.dist ? it.dist : it.n;
+        in_ext = std::max(in_ext, it.in_off + (it.batch - 1) * dist + it.n);
+        out_ext = std::max(out_ext, it.out_off + (it.batch - 1) * dist + it.n);
+        flops += 5.0 * (double)it.n * std::log2(std::max<double>((double)it.n, 2.0)) * (double)it.batch;
+        plan->n_transforms += it.batch;
+        if (size_index.count(it.n)) continue;
+        SizeInfo si;
+        const int64_t L = smooth13(it.n) ? it.n : bluestein_length(it.n);
+        const bool small = next_pow2((L + 7) / 8) <= 256 && (size_t)(lds_pad((int)L) + 2) * esize <= small_budget / 2;
+        if (!batch_size_desc(tb, prec, it.n, small ? 8 : 16, small ? 256 : 1024, small ? small_budget : big_budget,
+                             si, err)) {
+            delete plan;
+            return nullptr;
+        }
+        si.cls = small ? 0 : 1;
+        size_index[it.n] = (int)sizes.size();
+        sizes.push_back(si);
+    }
+    plan->in_extent_bytes = in_ext * esize;
+    plan->out_extent_bytes = out_ext * esize;
+    plan->flops = flops;
+    // tiles per class, cost-sorted (largest first)
+    std::vector<std::vector<std::pair<double, BatchTile>>> tiles(2);
+    for (const BatchItem& it : items) {
+        if (it.batch == 0) continue;
+        const SizeInfo& si = sizes[(size_t)size_index[it.n]];
+        const int64_t dist = it.dist ? it.dist : it.n;
+        for (int64_t l0 = 0; l0 < it.batch; l0 += si.C) {
+            BatchTile bt{};
+            bt.desc = (uint32_t)size_index[it.n];
+            bt.nlines = (uint32_t)std::min<int64_t>(si.C, it.batch - l0);
+            bt.bluestein = si.bluestein ? 1u : 0u;
+            bt.in_off = it.in_off + l0 * dist;
+            bt.out_off = it.out_off + l0 * dist;
+            bt.dist = dist;
+            const double cost = (double)bt.nlines * si.L * (std::log2((double)si.L + 1.0) + 8.0) *
+                                (si.bluestein ? 2.5 : 1.0);
+            tiles[(size_t)si.cls].push_back({cost, bt});
+        }
+    }
+    // direction: every tile is a complete transform; backward conjugates in
+    // and out (the Bluestein body handles its own conjugation)
+    for (SizeInfo& si : sizes)
+        if (dir == 1) si.d.flags |= kConjIn | kConjOut;
+    std::vector<PassDesc> descs;
+    for (const SizeInfo& si : sizes) descs.push_back(si.d);
+    plan->descs_off = tb.add(descs.data(), std::max<size_t>(1, descs.size()) * sizeof(PassDesc));
+    for (int c = 0; c < 2; c++) {
+        auto& tl = tiles[(size_t)c];
+        std::stable_sort(tl.begin(), tl.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
+        BatchClass bc;
+        bc.E = c == 0 ? 8 : 16;
+        bc.maxt = c == 0 ? 256 : 1024;
+        bc.ntiles = (int)tl.size();
+        size_t lds = 0;
+        for (const SizeInfo& si : sizes)
+            if (si.cls == c) lds = std::max(lds, (size_t)si.C * si.d.LP * esize);
+        bc.lds_bytes = (uint32_t)(lds + kBatchHdrBytes);
+        const int per_cu = c == 0 ? env_int("FFTZ_BATCH_WG_SMALL", 8) : env_int("FFTZ_BATCH_WG_BIG", 1);
+        bc.grid = (uint32_t)std::max<int64_t>(1, std::min<int64_t>(bc.ntiles, (int64_t)dev.cus * per_cu));
+        if (bc.ntiles) {
+            std::vector<BatchTile> v;
+            v.reserve(tl.size());
+            for (auto& t : tl) v.push_back(t.second);
+            bc.tiles_off = tb.add(v.data(), v.size() * sizeof(BatchTile));
+        }
+        plan->bclasses.push_back(bc);
+    }
+    std::ostringstream os;
+    os << "uneven batch: " << plan->n_transforms << " transforms, " << sizes.size() << " lengths; tiles small="
+       << plan->bclasses[0].ntiles << " (grid " << plan->bclasses[0].grid << ", lds " << plan->bclasses[0].lds_bytes
+       << ") large=" << plan->bclasses[1].ntiles << " (grid " << plan->bclasses[1].grid << ", lds "
+       << plan->bclasses[1].lds_bytes << "); ";
+    plan->description = os.str();
+    if (dry_run) return plan;
+    if (hipMalloc(&plan->tables, plan->host_tables.size()) != hipSuccess) {
+        err = "device allocation for batch tables failed";
+        delete plan;
+        return nullptr;
+    }
+    // resolve descriptor table offsets to device pointers, then upload
+    char* base = (char*)plan->tables;
+    PassDesc* hd = reinterpret_cast<PassDesc*>(plan->host_tables.data() + plan->descs_off);
+    for (size_t i = 0; i < sizes.size(); i++) {
+        hd[i].stw = base + (intptr_t)hd[i].stw;
+        if (sizes[i].bluestein) {
+            hd[i].chirp = base + (intptr_t)hd[i].chirp;
+            hd[i].spec = base + (intptr_t)hd[i].spec;
+        }
+    }
+    if (hipMemcpy(plan->tables, plan->host_tables.data(), plan->host_tables.size(), hipMemcpyHostToDevice) !=
+        hipSuccess) {
+        err = "upload of batch tables failed";
+        (void)hipFree(plan->tables);
+        delete plan;
+        return nullptr;
+    }
+    for (int c = 0; c < 2; c++)
+        (void)hipFuncSetAttribute(kernel_symbol_batch(prec, plan->bclasses[(size_t)c].E, plan->bclasses[(size_t)c].maxt),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)dev.lds_per_block);
+    plan->host_tables.clear();
+    plan->host_tables.shrink_to_fit();
+    plan->use_graph = env_int("FFTZ_BATCH_GRAPH", 1) != 0;
+    return plan;
+}
+
 Plan* plan_create(const ProblemSpec& spec_in, std::string& err, bool dry_run)
 {
     Plan* plan = new Plan();
@@ -878,8 +1117,44 @@ Plan* plan_create(const ProblemSpec& spec_in, std::string& err, bool dry_run)
     return plan;
 }
 
+static std::string batch_json(const Plan* p)
+{
+    std::ostringstream os;
+    os << "{\"batch\":1,\"prec\":" << p->spec.prec_bytes << ",\"in_extent_bytes\":" << p->in_extent_bytes
+       << ",\"out_extent_bytes\":" << p->out_extent_bytes << ",\"sizes\":[";
+    const PassDesc* hd = p->host_tables.empty()
+                             ? nullptr
+                             : reinterpret_cast<const PassDesc*>(p->host_tables.data() + p->descs_off);
+    size_t nsz = 0;
+    for (const BatchClass& bc : p->bclasses) {
+        const BatchTile* t = hd ? reinterpret_cast<const BatchTile*>(p->host_tables.data() + bc.tiles_off) : nullptr;
+        for (int i = 0; hd && i < bc.ntiles; i++) nsz = std::max(nsz, (size_t)t[i].desc + 1);
+    }
+    for (size_t i = 0; hd && i < nsz; i++) {
+        const PassDesc& d = hd[i];
+        os << (i ? "," : "") << "{\"L\":" << d.L << ",\"nvalid\":" << d.nvalid_in << ",\"C\":" << d.C
+           << ",\"TL\":" << d.TL << ",\"LP\":" << d.LP << ",\"flags\":" << d.flags << ",\"radix\":[";
+        for (int s = 0; s < d.nst; s++) os << (s ? "," : "") << d.radix[s];
+        os << "]}";
+    }
+    os << "],\"classes\":[";
+    for (size_t c = 0; c < p->bclasses.size(); c++) {
+        const BatchClass& bc = p->bclasses[c];
+        os << (c ? "," : "") << "{\"E\":" << bc.E << ",\"maxt\":" << bc.maxt << ",\"lds\":" << bc.lds_bytes
+           << ",\"grid\":" << bc.grid << ",\"tiles\":[";
+        const BatchTile* t = hd ? reinterpret_cast<const BatchTile*>(p->host_tables.data() + bc.tiles_off) : nullptr;
+        for (int i = 0; t && i < bc.ntiles; i++)
+            os << (i ? "," : "") << "[" << t[i].desc << "," << t[i].nlines << "," << t[i].bluestein << ","
+               << t[i].in_off << "," << t[i].out_off << "," << t[i].dist << "]";
+        os << "]}";
+    }
+    os << "]}";
+    return os.str();
+}
+
 std::string plan_json(const Plan* p)
 {
+    if (p->is_batch) return batch_json(p);
     std::ostringstream os;
     os << "{\"noop\":" << (p->noop ? 1 : 0) << ",\"work_bytes\":" << p->work_bytes
        << ",\"in_extent_bytes\":" << p->in_extent_bytes << ",\"out_extent_bytes\":" << p->out_extent_bytes
@@ -913,6 +1188,7 @@ static void slot_free(Slot* s)
     if (s->work) (void)hipFree(s->work);
     if (s->stage_in) (void)hipFree(s->stage_in);
     if (s->stage_out) (void)hipFree(s->stage_out);
+    if (s->counters) (void)hipFree(s->counters);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
 }
@@ -945,6 +1221,7 @@ Slot* plan_acquire_slot(Plan* p, hipError_t* err)
         e = hipMalloc(&s->work, p->work_bytes);
         s->work_bytes = p->work_bytes;
     }
+    if (e == hipSuccess && p->is_batch) e = hipMalloc((void**)&s->counters, 64);
     if (e != hipSuccess) {
         *err = e;
         slot_free(s);
@@ -978,8 +1255,39 @@ static hipError_t record_prof(Plan* p, hipStream_t stream)
     return hipEventRecord(ev, stream);
 }
 
+static hipError_t launch_batch(Plan* p, Slot* s, const void* in, void* out, hipStream_t stream)
+{
+    // counters are cleared by a kernel (not a memset node) so the sequence
+    // can be captured in a graph and replayed
+    int nc = (int)p->bclasses.size();
+    void* rargs[] = {&s->counters, &nc};
+    hipError_t e = hipLaunchKernel(kernel_symbol_reset(), dim3(1), dim3(64), rargs, 0, stream);
+    if (e != hipSuccess) return e;
+    if (p->profile && (e = record_prof(p, stream)) != hipSuccess) return e;
+    const char* base = (const char*)p->tables;
+    for (size_t c = 0; c < p->bclasses.size(); c++) {
+        const BatchClass& bc = p->bclasses[c];
+        if (bc.ntiles > 0) {
+            BatchArgs a;
+            a.descs = reinterpret_cast<const PassDesc*>(base + p->descs_off);
+            a.tiles = reinterpret_cast<const BatchTile*>(base + bc.tiles_off);
+            a.ntiles = bc.ntiles;
+            a.counter = s->counters + c;
+            a.in = in;
+            a.out = out;
+            void* args[] = {&a};
+            e = hipLaunchKernel(kernel_symbol_batch(p->spec.prec_bytes, bc.E, bc.maxt), dim3(bc.grid),
+                                dim3(bc.maxt), args, bc.lds_bytes, stream);
+            if (e != hipSuccess) return e;
+        }
+        if (p->profile && (e = record_prof(p, stream)) != hipSuccess) return e;
+    }
+    return hipGetLastError();
+}
+
 static hipError_t launch_all(Plan* p, Slot* s, const void* in, void* out, hipStream_t stream)
 {
+    if (p->is_batch) return launch_batch(p, s, in, out, stream);
     if (p->profile) {
         hipError_t e = record_prof(p, stream);
         if (e != hipSuccess) return e;
@@ -1016,7 +1324,7 @@ int plan_profile_read(Plan* p, float* ms, int cap)
         std::lock_guard<std::mutex> g(p->mu);
         evs.swap(p->prof_events);
     }
-    const size_t per = p->launches.size() + 1;  // events per execute
+    const size_t per = (p->is_batch ? p->bclasses.size() : p->launches.size()) + 1;  // events per execute
     int n = 0;
     for (size_t base = 0; base + per <= evs.size(); base += per) {
         for (size_t i = 0; i + 1 < per; i++) {

@@ -35,6 +35,21 @@ struct ProblemSpec {
     std::vector<AxisSpec> vecs;  // batch axes
 };
 
+// One uneven-batch entry: `batch` transforms of length n, `dist` elements
+// apart, starting at element offsets in_off / out_off of the caller buffers.
+struct BatchItem {
+    int64_t n, batch, in_off, out_off, dist;
+};
+
+// A persistent-kernel class of the uneven batch (small or large tiles).
+struct BatchClass {
+    int E = 8, maxt = 256;
+    uint32_t lds_bytes = 0;
+    uint32_t grid = 0;
+    int ntiles = 0;
+    int64_t tiles_off = -1;  // BatchTile array in the plan arena
+};
+
 struct Launch {
     KernelKind kind = KernelKind::Pass;
     bool all_radix = false;
@@ -61,6 +76,7 @@ struct Slot {
     void* stage_out = nullptr;
     size_t stage_in_bytes = 0, stage_out_bytes = 0;
     bool busy = false;
+    unsigned* counters = nullptr;  // uneven-batch work-queue counters
     // graph cache for the last (in, out) pair run on this slot
     hipGraphExec_t graph = nullptr;
     const void* graph_in = nullptr;
@@ -88,6 +104,12 @@ struct Plan {
     std::vector<hipEvent_t> prof_events;  // recorded, not yet read
     std::vector<hipEvent_t> prof_pool;    // free events
     std::string description;            // human-readable plan summary
+    // uneven batch (aoclfftz_gpu_batch_setup): size-class descriptors and
+    // per-class tile queues in the arena
+    bool is_batch = false;
+    int64_t descs_off = -1;
+    std::vector<BatchClass> bclasses;
+    int64_t n_transforms = 0;
     double flops = 0.0;                 // 5 N log2 N per transform x batch (C2C)
 };
 
@@ -95,6 +117,10 @@ struct Plan {
 // dry_run: build the launch list only (no device memory, no HIP calls), for
 // host-side inspection of the plan geometry.
 Plan* plan_create(const ProblemSpec& spec, std::string& err, bool dry_run = false);
+
+// Uneven batch of 1-D C2C transforms in one persistent launch per tile class.
+Plan* batch_plan_create(const std::vector<BatchItem>& items, int prec_bytes, int dir, std::string& err,
+                        bool dry_run = false);
 
 // JSON description of every launch of a plan (geometry, strides, flags).
 std::string plan_json(const Plan* p);

@@ -68,8 +68,14 @@ EXPORTS = (
     "aoclfftz_gpu_set_stream", "aoclfftz_gpu_synchronize", "aoclfftz_gpu_set_graph",
     "aoclfftz_gpu_plan_info", "aoclfftz_gpu_plan_flops", "aoclfftz_gpu_plan_device_bytes",
     "aoclfftz_gpu_plan_json", "aoclfftz_gpu_profile", "aoclfftz_gpu_profile_read",
-    "aoclfftz_gpu_plan_launches", "aoclfftz_gpu_execute_many",
+    "aoclfftz_gpu_plan_launches", "aoclfftz_gpu_execute_many", "aoclfftz_gpu_batch_setup",
+    "aoclfftz_gpu_batch_plan_json",
 )
+
+
+class BatchItem(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64), ("batch", ctypes.c_int64), ("in_offset", ctypes.c_int64),
+                ("out_offset", ctypes.c_int64), ("dist", ctypes.c_int64)]
 
 _lib = None
 
@@ -107,6 +113,12 @@ def load():
     lib.aoclfftz_gpu_plan_flops.restype = ctypes.c_double
     lib.aoclfftz_gpu_plan_json.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_char_p, ctypes.c_int64]
     lib.aoclfftz_gpu_plan_json.restype = ctypes.c_int64
+    lib.aoclfftz_gpu_batch_setup.argtypes = [ctypes.POINTER(BatchItem), ctypes.c_int64, ctypes.c_int32,
+                                             ctypes.c_int32]
+    lib.aoclfftz_gpu_batch_setup.restype = vp
+    lib.aoclfftz_gpu_batch_plan_json.argtypes = [ctypes.POINTER(BatchItem), ctypes.c_int64, ctypes.c_int32,
+                                                 ctypes.c_int32, ctypes.c_char_p, ctypes.c_int64]
+    lib.aoclfftz_gpu_batch_plan_json.restype = ctypes.c_int64
     lib.aoclfftz_gpu_execute_many.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.c_int32]
     lib.aoclfftz_gpu_execute_many.restype = ctypes.c_int
     lib.aoclfftz_gpu_profile.argtypes = [vp, ctypes.c_int32]
@@ -237,6 +249,39 @@ class Plan:
             self.destroy()
         except Exception:
             pass
+
+
+def _items(items):
+    arr = (BatchItem * max(1, len(items)))()
+    for i, it in enumerate(items):
+        n, b, io, oo = it[:4]
+        arr[i] = BatchItem(n, b, io, oo, it[4] if len(it) > 4 else 0)
+    return arr
+
+
+class BatchPlan(Plan):
+    """Uneven batch: items are (n, batch, in_offset, out_offset[, dist])."""
+
+    def __init__(self, items, precision: str = "f", direction: int = 0):
+        self.precision = precision
+        self._arr = _items(items)
+        self.handle = load().aoclfftz_gpu_batch_setup(self._arr, len(items), 4 if precision == "f" else 8,
+                                                      direction)
+        if not self.handle:
+            raise RuntimeError("aoclfftz_gpu_batch_setup failed")
+
+
+def batch_plan_json(items, precision: str = "f", direction: int = 0) -> dict:
+    import json
+    lib = load()
+    arr = _items(items)
+    pb = 4 if precision == "f" else 8
+    n = lib.aoclfftz_gpu_batch_plan_json(arr, len(items), pb, direction, None, 0)
+    if n < 0:
+        raise ValueError("batch plan rejected")
+    buf = ctypes.create_string_buffer(int(n) + 1)
+    lib.aoclfftz_gpu_batch_plan_json(arr, len(items), pb, direction, buf, n + 1)
+    return json.loads(buf.value.decode())
 
 
 def plan_json(dims, vecs, precision: str = "f", direction: int = 0, fft_type: int = 0,

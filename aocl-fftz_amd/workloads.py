@@ -1,0 +1,57 @@
+"""Seeded synthetic workloads for the benchmark configurations.
+
+uneven_batch(): 1-D C2C items with lengths drawn from a seed -- 70% from the
+7-smooth numbers 2^a 3^b 5^c 7^d in [16, 16384], 30% from the primes in
+[101, 8191] (the Bluestein path) -- each with a batch count in 1..64, packed
+contiguously (input and output offsets are the running element count).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def smooth7(lo: int = 16, hi: int = 16384):
+    out = []
+    a = 1
+    while a <= hi:
+        b = a
+        while b <= hi:
+            c = b
+            while c <= hi:
+                d = c
+                while d <= hi:
+                    if d >= lo:
+                        out.append(d)
+                    d *= 7
+                c *= 5
+            b *= 3
+        a *= 2
+    return sorted(out)
+
+
+def primes(lo: int = 101, hi: int = 8191):
+    sieve = np.ones(hi + 1, dtype=bool)
+    sieve[:2] = False
+    for p in range(2, int(hi ** 0.5) + 1):
+        if sieve[p]:
+            sieve[p * p::p] = False
+    return [int(p) for p in np.nonzero(sieve)[0] if p >= lo]
+
+
+def uneven_batch(count: int = 20000, seed: int = 3, max_batch: int = 64, smooth_frac: float = 0.7):
+    """List of (n, batch, in_offset, out_offset) packed contiguously."""
+    rng = np.random.default_rng(seed)
+    sm = np.array(smooth7())
+    pr = np.array(primes())
+    items = []
+    off = 0
+    for _ in range(count):
+        n = int(rng.choice(sm)) if rng.random() < smooth_frac else int(rng.choice(pr))
+        b = int(rng.integers(1, max_batch + 1))
+        items.append((n, b, off, off))
+        off += n * b
+    return items, off
+
+
+def uneven_flops(items) -> float:
+    return float(sum(5.0 * n * np.log2(n) * b for n, b, *_ in items))

@@ -76,7 +76,7 @@ def roofline_from_profile(prof, bytes_per_launch):
 
 
 def bench_roundtrip_2p24(torch, fftz, steps, warmup, n=1 << 24):
-    """N=2^24 complex128 in place, forward + backward per step."""
+    """Large 1-D: N=2^24 complex128 in place, forward + backward per step."""
     g = torch.Generator(device="cuda")
     g.manual_seed(1)
     x = torch.randn(n, dtype=torch.complex128, device="cuda", generator=g)
@@ -144,7 +144,7 @@ def bench_roundtrip_2p24(torch, fftz, steps, warmup, n=1 << 24):
 
 
 def bench_batched_n1024(torch, fftz, steps, warmup, n=1024, b=4096):
-    """N=1024 complex64, batch 4096, out of place."""
+    """Batched: N=1024 complex64, batch 4096, out of place."""
     g = torch.Generator(device="cuda")
     g.manual_seed(1)
     nbuf = 8  # 8 x (32 + 32) MiB = 512 MiB > Infinity Cache

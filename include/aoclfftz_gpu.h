@@ -69,6 +69,34 @@ FFTZ_INT32 aoclfftz_gpu_plan_launches(FFTZ_VOID *handle);
 FFTZ_INT64 aoclfftz_gpu_plan_json(FFTZ_VOID *problem, FFTZ_INT32 precision_bytes, FFTZ_INT32 ilp64,
                                   FFTZ_CHAR *buf, FFTZ_INT64 buflen);
 
+/* Uneven batch: many 1-D C2C transforms of different lengths in one
+ * persistent launch per tile class, fed by a device work queue.  Item i is
+ * `batch` transforms of length n, `dist` elements apart (0 means n), starting
+ * at element offsets in_offset / out_offset of the buffers later passed to
+ * aoclfftz_execute_io (complex elements; precision_bytes 4 or 8;
+ * direction 0 forward, 1 backward, unnormalised).  Lengths with a prime
+ * factor above 13 run Bluestein inside the same launch.  The handle works
+ * with aoclfftz_execute_io, aoclfftz_destroy and the stream / graph /
+ * profiling calls above.  No counterpart in the reference, which plans and
+ * executes each (n, batch) separately. */
+typedef struct
+{
+    FFTZ_INT64 n;
+    FFTZ_INT64 batch;
+    FFTZ_INT64 in_offset;
+    FFTZ_INT64 out_offset;
+    FFTZ_INT64 dist;
+} aoclfftz_gpu_batch_item;
+
+FFTZ_VOID *aoclfftz_gpu_batch_setup(const aoclfftz_gpu_batch_item *items, FFTZ_INT64 count,
+                                    FFTZ_INT32 precision_bytes, FFTZ_INT32 direction);
+
+/* Dry-run of aoclfftz_gpu_batch_setup (no GPU needed): JSON with the size
+ * classes and every tile of the work queue.  Returns the JSON length or -1. */
+FFTZ_INT64 aoclfftz_gpu_batch_plan_json(const aoclfftz_gpu_batch_item *items, FFTZ_INT64 count,
+                                        FFTZ_INT32 precision_bytes, FFTZ_INT32 direction, FFTZ_CHAR *buf,
+                                        FFTZ_INT64 buflen);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,6 +1,6 @@
 /*
  * fft_oracle.c -- CPU restatement of the reference's transform, used ONLY as
- * the checker for the GPU library (tests, smoke check, CPU baseline leg of the
+ * the expected-result source for the GPU library (tests, smoke check, CPU baseline leg of the
  * bench).  Nothing in the product library links or calls this code.
  *
  * Everything is computed in double (long double for the naive DFT).  Each
@@ -121,30 +121,45 @@ static void fft_rec(int64_t n, const cd *in, int64_t is, cd *out, int64_t os,
                     int sign);
 
 /* Cached tables W[p] = exp(sign 2 pi i p / n), p in [0, n), built with
- * unit_root (octant-folded, ~1 ulp).  Small FIFO cache; the oracle runs
- * single-threaded. */
-#define ROOT_CACHE 48
-static struct { int64_t n; int sign; cd *w; } g_roots[ROOT_CACHE];
-static int g_roots_next = 0;
+ * unit_root (octant-folded, ~1 ulp).  A table handed out stays valid until
+ * the exported call that asked for it returns: nothing is evicted inside a
+ * transform (a recursive transform holds its table across the sub-transforms
+ * and leaf DFTs that fetch theirs).  roots_trim(), at the entry of each
+ * exported transform, drops the whole cache once it exceeds a byte budget.
+ * The oracle runs single-threaded. */
+#define ROOT_CACHE_BYTES ((size_t)1 << 29)
+static struct root_entry { int64_t n; int sign; cd *w; } *g_roots = NULL;
+static size_t g_roots_len = 0, g_roots_cap = 0, g_roots_bytes = 0;
 
 static const cd *roots(int64_t n, int sign)
 {
-    for (int i = 0; i < ROOT_CACHE; i++)
-        if (g_roots[i].w && g_roots[i].n == n && g_roots[i].sign == sign) return g_roots[i].w;
+    for (size_t i = 0; i < g_roots_len; i++)
+        if (g_roots[i].n == n && g_roots[i].sign == sign) return g_roots[i].w;
     cd *w = (cd *)malloc(sizeof(cd) * (size_t)n);
     for (int64_t p = 0; p < n; p++) w[p] = unit_root(p, n, sign);
-    int slot = g_roots_next;
-    g_roots_next = (g_roots_next + 1) % ROOT_CACHE;
-    free(g_roots[slot].w);
-    g_roots[slot].n = n;
-    g_roots[slot].sign = sign;
-    g_roots[slot].w = w;
+    if (g_roots_len == g_roots_cap) {
+        g_roots_cap = g_roots_cap ? 2 * g_roots_cap : 64;
+        g_roots = (struct root_entry *)realloc(g_roots, sizeof(*g_roots) * g_roots_cap);
+    }
+    g_roots[g_roots_len].n = n;
+    g_roots[g_roots_len].sign = sign;
+    g_roots[g_roots_len].w = w;
+    g_roots_len++;
+    g_roots_bytes += sizeof(cd) * (size_t)n;
     return w;
 }
 
 void orc_clear_cache(void)
 {
-    for (int i = 0; i < ROOT_CACHE; i++) { free(g_roots[i].w); g_roots[i].w = NULL; }
+    for (size_t i = 0; i < g_roots_len; i++) free(g_roots[i].w);
+    g_roots_len = 0;
+    g_roots_bytes = 0;
+}
+
+/* Only called where no table is in use (entry of an exported transform). */
+static void roots_trim(void)
+{
+    if (g_roots_bytes > ROOT_CACHE_BYTES) orc_clear_cache();
 }
 
 /* Small DFT by definition, in double, for leaf sizes (<= 16 or 20/48 in the
@@ -236,7 +251,6 @@ static void fft_rec(int64_t n, const cd *in, int64_t is, cd *out, int64_t os,
             dft_small(r, t, 1, o, 1, sign);
         } else {
             fft_rec(r, t, 1, o, 1, sign);
-            w = roots(n, sign); /* the sub-transform may have cycled the cache */
         }
         for (int64_t k2 = 0; k2 < r; k2++) out[(k1 + m * k2) * os] = o[k2];
     }
@@ -247,6 +261,7 @@ static void fft_rec(int64_t n, const cd *in, int64_t is, cd *out, int64_t os,
 void orc_fft1d(int64_t n, const double *in, int64_t is, double *out, int64_t os,
                int sign)
 {
+    roots_trim();
     cd *src = (cd *)malloc(sizeof(cd) * (size_t)n);
     cd *dst = (cd *)malloc(sizeof(cd) * (size_t)n);
     for (int64_t j = 0; j < n; j++) { src[j].re = in[2 * j * is]; src[j].im = in[2 * j * is + 1]; }
@@ -286,6 +301,7 @@ static int64_t offset_of(int64_t lin, const int64_t *t, int rank, int which)
 void orc_c2c(const double *in, double *out, int dim_rank, const int64_t *dims,
              int vec_rank, const int64_t *vecs, int sign)
 {
+    roots_trim();
     int64_t n = prod_n(dims, dim_rank);
     int64_t nb = prod_n(vecs, vec_rank);
     if (n == 0 || nb == 0) return;
@@ -336,6 +352,7 @@ void orc_c2c(const double *in, double *out, int dim_rank, const int64_t *dims,
  * ------------------------------------------------------------------------ */
 void orc_r2c_packed_1d(int64_t n, const double *in, double *out)
 {
+    roots_trim();
     int64_t M = n / 2;
     cd *z = (cd *)malloc(sizeof(cd) * (size_t)M);
     cd *x = (cd *)malloc(sizeof(cd) * (size_t)M);
@@ -366,6 +383,7 @@ void orc_r2c_packed_1d(int64_t n, const double *in, double *out)
 /* R2C by definition (real input promoted to complex), any n, n/2+1 bins. */
 void orc_r2c_1d(int64_t n, const double *in, int64_t is, double *out, int64_t os)
 {
+    roots_trim();
     cd *x = (cd *)malloc(sizeof(cd) * (size_t)n);
     cd *z = (cd *)malloc(sizeof(cd) * (size_t)n);
     for (int64_t j = 0; j < n; j++) { x[j].re = in[j * is]; x[j].im = 0.0; }
@@ -379,6 +397,7 @@ void orc_r2c_1d(int64_t n, const double *in, int64_t is, double *out, int64_t os
  * are zero (docs/rst/data_layout_conventions.rst:49-60). */
 void orc_c2r_1d(int64_t n, const double *in, int64_t is, double *out, int64_t os)
 {
+    roots_trim();
     cd *x = (cd *)malloc(sizeof(cd) * (size_t)n);
     cd *z = (cd *)malloc(sizeof(cd) * (size_t)n);
     for (int64_t k = 0; k <= n / 2; k++) { x[k].re = in[2 * k * is]; x[k].im = in[2 * k * is + 1]; }

@@ -1,7 +1,7 @@
 """CPU oracle for the GPU FFT library -- test infrastructure only.
 
 Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg may import
-this module, and only as the checker.  It wraps two things:
+this module, and only to compute expected results.  It wraps two things:
 
 * ``liboracle.so`` (oracle/fft_oracle.c): a plain-C FP64 restatement of the
   reference's transform -- recursive mixed-radix Cooley-Tukey

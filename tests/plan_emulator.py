@@ -146,3 +146,33 @@ def check_launch_limits(plan, lds_cap=163840):
             tiles.append(t0)
         assert sorted(tiles) == list(range(ntiles)), "tile remap not a permutation"
         assert ln["nlines"] < 2 ** 31
+
+
+def run_batch(plan, x, direction=0):
+    """Replay an uneven-batch work queue (aoclfftz_gpu_batch_plan_json) on a
+    flat complex128 input; returns the output buffer.  Checks that tiles
+    cover every transform exactly once and stay inside the buffers."""
+    esz = 2 * plan["prec"]
+    nin = plan["in_extent_bytes"] // esz
+    nout = plan["out_extent_bytes"] // esz
+    assert x.size >= nin
+    y = np.zeros(nout, np.complex128)
+    seen = set()
+    sizes = plan["sizes"]
+    for cls in plan["classes"]:
+        for desc, nl, blu, ioff, ooff, dist in cls["tiles"]:
+            s = sizes[desc]
+            n = s["nvalid"]
+            assert nl <= s["C"] and s["C"] * s["TL"] <= cls["maxt"]
+            assert s["C"] * s["LP"] * esz + 768 <= cls["lds"] <= 163840
+            assert bool(blu) == (s["L"] != n)
+            for l in range(nl):
+                i0 = ioff + l * dist
+                o0 = ooff + l * dist
+                _check(np.array([i0, i0 + n - 1]), nin, "batch load")
+                _check(np.array([o0, o0 + n - 1]), nout, "batch store")
+                assert o0 not in seen, "transform written twice"
+                seen.add(o0)
+                seg = x[i0:i0 + n]
+                y[o0:o0 + n] = np.fft.fft(seg) if direction == 0 else np.fft.ifft(seg) * n
+    return y, seen

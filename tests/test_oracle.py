@@ -1,4 +1,4 @@
-"""Pin the CPU oracle (oracle/fft_oracle.c) before trusting it as the checker.
+"""Pin the CPU oracle (oracle/fft_oracle.c) before trusting its expected results.
 
 The reference holds no golden vectors or known-answer files for this path
 (its tests compute everything on the fly: naive DFT, property tests,
@@ -48,6 +48,20 @@ def test_oracle_vs_numpy(n):
     x = rnd(n, 7)
     assert O.rel_l2(O.fft(x), np.fft.fft(x)) < 2e-15 * np.log2(n)
     assert O.rel_l2(O.fft(x, +1), np.fft.ifft(x) * n) < 2e-15 * np.log2(n)
+
+
+def test_oracle_many_lengths_in_one_process():
+    """Root tables fetched by a transform stay valid through its leaf DFTs
+    even after hundreds of other lengths have gone through the table cache
+    (a bounded cache that evicted mid-transform gave wrong odd bins for
+    n = 10584 = 12 * 7 * 7 * 6 * 3 after ~250 other lengths)."""
+    import workloads
+    items, _ = workloads.uneven_batch(300, seed=21, max_batch=8)
+    O.clear_cache()
+    for i, (n, b, _, _) in enumerate(items):
+        x = rnd((b, n), i)
+        err = O.rel_l2(O.fft(x), np.fft.fft(x, axis=1))
+        assert err < 2e-15 * np.log2(n), (i, n, b, err)
 
 
 @pytest.mark.parametrize("n", [8, 101, 1024, 4096])
