@@ -448,7 +448,8 @@ const FFTZ_CHAR* aoclfftz_gpu_plan_info(FFTZ_VOID* handle)
     if (!h) return "";
     std::ostringstream os;
     fftz::Plan* p = h->plan;
-    os << (p->spec.prec_bytes == 4 ? "fp32" : "fp64") << " " << p->description << "passes=" << p->launches.size();
+    os << (p->spec.prec_bytes == 4 ? "fp32" : "fp64") << " " << p->description;
+    if (!p->is_batch) os << "passes=" << p->launches.size();
     for (const auto& ln : p->launches) {
         os << " [" << (ln.kind == fftz::KernelKind::Bluestein ? "bluestein" : (ln.fixed ? "fixed" : "pass"))
            << " L=" << ln.d.L << " r=";
@@ -457,7 +458,6 @@ const FFTZ_CHAR* aoclfftz_gpu_plan_info(FFTZ_VOID* handle)
            << " lds=" << ln.lds_bytes << ((ln.d.flags & fftz::kInCol) ? " incol" : "")
            << ((ln.d.flags & fftz::kOutCol) ? " outcol" : "") << ((ln.d.flags & fftz::kTw4) ? " tw4" : "") << "]";
     }
-    if (p->is_batch) os << " " << p->description;
     os << " work=" << p->work_bytes;
     h->info = os.str();
     return h->info.c_str();

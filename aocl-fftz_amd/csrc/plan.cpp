@@ -1216,7 +1216,9 @@ Slot* plan_acquire_slot(Plan* p, hipError_t* err)
             return s;
         }
     Slot* s = new Slot();
-    hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    // a blocking stream: a synchronous execute is ordered after work still
+    // pending on the null stream (where most callers produced their buffers)
+    hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamDefault);
     if (e == hipSuccess && p->work_bytes) {
         e = hipMalloc(&s->work, p->work_bytes);
         s->work_bytes = p->work_bytes;

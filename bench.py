@@ -12,6 +12,8 @@ Also measured in the same run and reported in the same JSON line:
   * batched FP32 C2C, N = 1024, batch 4096, out of place (single LDS pass),
     with input buffers rotated over 512 MiB so the 256 MiB Infinity Cache does
     not serve them, plus the cache-warm figure;
+  * the uneven batch: 20,000 complex64 items of mixed length (7-smooth and
+    prime) with batch 1..64 each, in one persistent launch per tile class;
   * roofline of the dominant kernel: algorithmic bytes per launch over its
     average duration measured with HIP events on its own stream;
   * the reference's CPU path (AOCL-FFTZ built from its sources, OpenMP) timed
@@ -62,6 +64,14 @@ def launch_bytes(plan_json):
             nout = 2 * ln["L"]
         out.append(ln["nlines"] * (nin * in_e + nout * out_e))
     return out
+
+
+def kernel_name(launch, real):
+    """Kernel symbol (as rocprofv3 abbreviates it) of one plan launch."""
+    if launch["fixed"]:
+        return f"k_fixed_pass<{real}, {launch['L']}>"
+    kind = "k_bluestein" if launch["kind"] == "bluestein" else "k_fft_pass"
+    return f"{kind}<{real}, E={launch['E']}, {launch['block']}>"
 
 
 def roofline_from_profile(prof, bytes_per_launch):
@@ -124,6 +134,7 @@ def bench_roundtrip_2p24(torch, fftz, steps, warmup, n=1 << 24):
         "launch_bytes": bpl,
         "dominant_launch": kidx,
         "dominant_achieved_Bps": achieved,
+        "dominant_kernel": kernel_name(pj["launches"][kidx], "double"),
         "transform_frac_of_hbm": (2 * 16.0 * n * 2) / (ms * 1e-3) / HBM_PEAK,
         "plan": fwd.info,
     }
@@ -201,6 +212,61 @@ def bench_batched_n1024(torch, fftz, steps, warmup, n=1024, b=4096):
     res["_y"] = ys[0].cpu().numpy()
     p.destroy()
     del xs, ys
+    torch.cuda.empty_cache()
+    return res
+
+
+def bench_uneven(torch, fftz, steps, warmup):
+    """Uneven batch: 20,000 complex64 C2C items from seed 3 (70% 7-smooth
+    lengths in [16, 16384], 30% primes in [101, 8191] on the Bluestein path),
+    batch 1..64 each, packed contiguously, forward, out of place; one
+    persistent launch per tile class, replayed as a graph."""
+    import workloads
+    items, total = workloads.uneven_batch()
+    g = torch.Generator(device="cuda")
+    g.manual_seed(3)
+    x = torch.empty(total, dtype=torch.complex64, device="cuda")
+    x.view(torch.float32).uniform_(-1.0, 1.0, generator=g)
+    y = torch.empty_like(x)
+    stream = torch.cuda.current_stream()
+    t_plan = time.perf_counter()
+    p = fftz.BatchPlan(items, "f", 0)
+    t_plan = time.perf_counter() - t_plan
+    assert p.set_stream(stream) == 0
+    log(f"[bench] uneven plan ({t_plan * 1e3:.0f} ms): {p.info}")
+    for _ in range(max(warmup, 1)):
+        assert p.execute_io(x, y) == 0
+    torch.cuda.synchronize()
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    t0.record(stream)
+    for _ in range(steps):
+        assert p.execute_io(x, y) == 0
+    t1.record(stream)
+    torch.cuda.synchronize()
+    ms = t0.elapsed_time(t1) / steps
+    p.profile(True)
+    assert p.execute_io(x, y) == 0
+    torch.cuda.synchronize()
+    prof = p.profile_read()
+    p.profile(False)
+    flops = workloads.uneven_flops(items)
+    byts = 2 * 8.0 * total  # every element read once and written once
+    transforms = sum(it[1] for it in items)
+    res = {
+        "items": len(items),
+        "transforms": transforms,
+        "elements": total,
+        "ms_per_execute": ms,
+        "gflops": flops / (ms * 1e-3) / 1e9,
+        "transforms_per_s": transforms / (ms * 1e-3),
+        "frac_of_hbm": byts / (ms * 1e-3) / HBM_PEAK,
+        "class_ms": [round(v, 3) for v in prof[-1]] if prof else None,
+        "plan_ms": t_plan * 1e3,
+        "plan": p.info,
+    }
+    p.destroy()
+    del x, y
     torch.cuda.empty_cache()
     return res
 
@@ -296,6 +362,9 @@ def main():
     r10 = bench_batched_n1024(torch, fftz, args.steps, args.warmup)
     log(f"[bench] n1024: {r10['ms_per_execute'] * 1e3:.2f} us {r10['gflops']:.1f} GFLOP/s "
         f"frac {r10['frac_of_hbm']:.3f}")
+    log("[bench] start uneven batch (20,000 items) fp32")
+    ru = bench_uneven(torch, fftz, args.steps, args.warmup)
+    log(f"[bench] uneven: {ru['ms_per_execute']:.3f} ms {ru['gflops']:.1f} GFLOP/s frac {ru['frac_of_hbm']:.3f}")
     copy_bps = copy_ceiling(torch)
     log(f"[bench] copy ceiling {copy_bps / 1e9:.0f} GB/s")
     base, extra = (None, {})
@@ -320,8 +389,8 @@ def main():
                                "(four-step path)", "n": 1 << 24, "batch": 1, "transforms_per_step": 2},
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4),
-                     "traffic": _traffic("k_fixed_pass<double, 256>"),
-                     "kernel": f"k_fft_pass launch #{r24['dominant_launch']} of the four-step plan",
+                     "traffic": _traffic(r24["dominant_kernel"]),
+                     "kernel": f"{r24['dominant_kernel']}, launch #{r24['dominant_launch']} of the four-step plan",
                      "algorithmic_bytes_per_launch": r24["launch_bytes"][r24["dominant_launch"]]},
         "cpu_baseline": base,
         "roundtrip_rel_l2": r24["roundtrip_rel_l2"],
@@ -329,6 +398,7 @@ def main():
         "launch_ms_avg": r24["launch_ms_avg"],
         "batched_n1024_fp32": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r10.items()
                                if not k.startswith("_")},
+        "uneven_batch_fp32": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in ru.items()},
     }
     line["copy_ceiling_GBps"] = round(copy_bps / 1e9, 1)
     line.update(extra)
