@@ -442,6 +442,47 @@ aoclfftz_error_type aoclfftz_gpu_set_graph(FFTZ_VOID* handle, FFTZ_INT32 enable)
     return AOCLFFTZ_SUCCESS;
 }
 
+aoclfftz_error_type aoclfftz_gpu_set_compute_mode(FFTZ_VOID* handle, FFTZ_INT32 mode)
+{
+    Handle* h = as_handle(handle);
+    if (!h) return AOCLFFTZ_EXECUTION_FAILURE;
+    if (mode < fftz::kMfmaNone || mode > fftz::kMfmaF16x2 || h->plan->is_batch) {
+        report_error(__func__, "compute mode %d is not available for this handle", (int)mode);
+        return AOCLFFTZ_INVALID_INPUT;
+    }
+    std::lock_guard<std::mutex> g(h->amu);
+    fftz::Plan* old = h->plan;
+    if (old->spec.mfma_mode == mode) return AOCLFFTZ_SUCCESS;
+    fftz::ProblemSpec spec = old->spec;
+    spec.mfma_mode = mode;
+    std::string err;
+    fftz::Plan* np = fftz::plan_create(spec, err);
+    if (!np) {
+        report_error(__func__, "re-planning failed: %s", err.c_str());
+        return AOCLFFTZ_SETUP_FAILURE;
+    }
+    bool applies = mode == fftz::kMfmaNone;
+    for (const auto& ln : np->launches) applies = applies || ln.mfma == mode;
+    if (!applies) {
+        fftz::plan_destroy(np);
+        report_error(__func__, "compute mode %d applies to no pass of this plan (complex64, lengths 256 / 4096)",
+                     (int)mode);
+        return AOCLFFTZ_INVALID_INPUT;
+    }
+    np->setup_in = old->setup_in;
+    np->setup_out = old->setup_out;
+    np->use_graph = old->use_graph;
+    np->profile = old->profile;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    if (h->aslot) {
+        fftz::plan_release_slot(old, h->aslot);
+        h->aslot = nullptr;
+    }
+    fftz::plan_destroy(old);
+    h->plan = np;
+    return AOCLFFTZ_SUCCESS;
+}
+
 const FFTZ_CHAR* aoclfftz_gpu_plan_info(FFTZ_VOID* handle)
 {
     Handle* h = as_handle(handle);
@@ -451,7 +492,10 @@ const FFTZ_CHAR* aoclfftz_gpu_plan_info(FFTZ_VOID* handle)
     os << (p->spec.prec_bytes == 4 ? "fp32" : "fp64") << " " << p->description;
     if (!p->is_batch) os << "passes=" << p->launches.size();
     for (const auto& ln : p->launches) {
-        os << " [" << (ln.kind == fftz::KernelKind::Bluestein ? "bluestein" : (ln.fixed ? "fixed" : "pass"))
+        static const char* kMfmaName[] = {"", "mfma-f32", "mfma-bf16x3", "mfma-f16x2"};
+        os << " [" << (ln.kind == fftz::KernelKind::Bluestein ? "bluestein"
+                       : ln.mfma ? kMfmaName[ln.mfma & 3]
+                       : (ln.fixed ? "fixed" : "pass"))
            << " L=" << ln.d.L << " r=";
         for (int s = 0; s < ln.d.nst; s++) os << (s ? "x" : "") << ln.d.radix[s];
         os << " C=" << ln.d.C << " TL=" << ln.d.TL << " grid=" << ln.grid << " block=" << ln.block

@@ -24,4 +24,9 @@ const void* kernel_symbol_reset();
 // then remainder, 16 elements per thread, block <= 256); nullptr otherwise.
 const void* kernel_symbol_fixed(int prec_bytes, int L);
 
+// Matrix-core pass (kernels_mfma.hip) for complex64, L = 256 or 4096, with
+// operand precision `mode` (MfmaMode); nullptr for other arguments.
+// Kernel arguments: (PassDesc, const float* fragments).
+const void* kernel_symbol_mfma(int mode, int L);
+
 }  // namespace fftz

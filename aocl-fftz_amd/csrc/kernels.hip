@@ -193,9 +193,9 @@ __device__ __forceinline__ void stage(const PassDesc& p, cplx<R>* lds, int tid, 
 }
 
 template <typename R, bool ALLRAD, int E>
-__device__ __forceinline__ void run_stages(const PassDesc& p, cplx<R>* lds, int tid)
+__device__ __forceinline__ void run_stages(const PassDesc& p, cplx<R>* lds, int tid, int s0, int s1)
 {
-    for (int s = 0; s < p.nst; s++) {
+    for (int s = s0; s < s1; s++) {
         switch (p.radix[s]) {
             case 2: stage<R, E, 2>(p, lds, tid, s); break;
             case 4: stage<R, E, 4>(p, lds, tid, s); break;
@@ -324,6 +324,15 @@ __device__ __forceinline__ void store_tile(const PassDesc& p, const cplx<R>* lds
     }
 }
 
+// The pass descriptor read in place from the kernel-argument segment (the
+// kernels take it as their only argument, at offset 0).  A by-value copy
+// whose arrays are indexed by the runtime stage number would be materialised
+// in per-lane scratch memory.
+__device__ __forceinline__ const PassDesc& kernarg_desc()
+{
+    return *(const PassDesc*)__builtin_amdgcn_kernarg_segment_ptr();  // address-space cast
+}
+
 __device__ __forceinline__ uint32_t tile_of_block(const PassDesc& p)
 {
     uint32_t tile = blockIdx.x;
@@ -338,16 +347,236 @@ __device__ __forceinline__ uint32_t tile_of_block(const PassDesc& p)
     return tile;
 }
 
+// ---------------------------------------------------------------------------
+// Register-direct first and last stages (complex in / complex or real-pair
+// out, no real-packed step).  Stage 0 (ls = 1) of butterfly j reads
+// x[i*nbl + j], i < R -- coalesced across j -- so a thread loads exactly its
+// butterflies' operands from HBM, runs the DFT and writes y[j*R + i] to LDS.
+// The last stage (ls = nbl, one group) writes y[i*nbl + j] straight to HBM
+// with the fused four-step twiddle, scale and conjugation.  This drops the
+// LDS round trip and barrier of a staged load and of a staged store.
+// (c, t) is the side's thread mapping (row or column).
+// ---------------------------------------------------------------------------
+template <typename R>
+struct LineIo {
+    const cplx<R>* src;
+    cplx<R>* dst;
+    uint32_t l0;
+    bool live_in, live_out;
+};
+
+template <typename R, int E, int RAD>
+__device__ __forceinline__ void first_stage_direct(const PassDesc& p, cplx<R>* lds, int c, int t, const cplx<R>* src,
+                                                   bool live)
+{
+    constexpr int K = (E + RAD - 1) / RAD;
+    const int nbl = p.L / RAD;
+    const int nv = p.nvalid_in;
+    const int64_t ies = p.ies;
+    cplx<R> v[K][RAD];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int j = t + k * p.TL;
+        if (live && j < nbl) {
+#pragma unroll
+            for (int i = 0; i < RAD; i++) {
+                const int e = i * nbl + j;
+                v[k][i] = {R(0), R(0)};
+                if (e < nv) v[k][i] = src[(int64_t)e * ies];
+            }
+        }
+    }
+    cplx<R>* line = lds + c * p.LP;
+    const bool cj = (p.flags & kConjIn) != 0;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int j = t + k * p.TL;
+        if (live && j < nbl) {
+            if (cj) {
+#pragma unroll
+                for (int i = 0; i < RAD; i++) v[k][i].y = -v[k][i].y;
+            }
+            Dft<RAD, R>::run(v[k]);
+#pragma unroll
+            for (int i = 0; i < RAD; i++) line[lds_pad(j * RAD + i)] = v[k][i];
+        }
+    }
+}
+
+// Output side of the last stage for one butterfly: element e = i*nbl + j.
+template <typename R, int RAD>
+__device__ __forceinline__ void store_butterfly(const PassDesc& p, cplx<R>* v, int j, int nbl, cplx<R>* dst,
+                                                uint32_t l0)
+{
+    if (p.flags & kTw4) {
+        // w_N^(l0 e), e = i*nbl + j: two table products, then an FP64
+        // recurrence over i
+        cplx<double> wc = tw4_root(p, (int64_t)l0 * j);
+        const cplx<double> ws = tw4_root(p, (int64_t)l0 * nbl);
+#pragma unroll
+        for (int i = 0; i < RAD; i++) {
+            cplx<double> vd = {double(v[i].x), double(v[i].y)};
+            vd = cmul(vd, wc);
+            v[i] = {R(vd.x), R(vd.y)};
+            wc = cmul(wc, ws);
+        }
+    }
+    if (p.flags & kScale) {
+        const R sc = R(sizeof(R) == 4 ? p.scale_f : p.scale_d);
+#pragma unroll
+        for (int i = 0; i < RAD; i++) v[i] = cscale(v[i], sc);
+    }
+    if (p.flags & kConjOut) {
+#pragma unroll
+        for (int i = 0; i < RAD; i++) v[i].y = -v[i].y;
+    }
+    const int nvo = p.nvalid_out;
+    if (p.flags & kRealOut) {
+        R* d = reinterpret_cast<R*>(dst);
+#pragma unroll
+        for (int i = 0; i < RAD; i++) {
+            const int e = i * nbl + j;
+            if (e < nvo) {
+                d[(int64_t)(2 * e) * p.oes] = v[i].x;
+                d[(int64_t)(2 * e + 1) * p.oes] = v[i].y;
+            }
+        }
+        return;
+    }
+#pragma unroll
+    for (int i = 0; i < RAD; i++) {
+        const int e = i * nbl + j;
+        if (e < nvo) dst[(int64_t)e * p.oes] = v[i];
+    }
+}
+
+template <typename R, int E, int RAD>
+__device__ __forceinline__ void last_stage_direct(const PassDesc& p, const cplx<R>* lds, int c, int t, cplx<R>* dst,
+                                                  uint32_t l0, bool live, int s)
+{
+    constexpr int K = (E + RAD - 1) / RAD;
+    const int nbl = p.L / RAD;  // = ls of the last stage; g = 0, cc = j
+    const cplx<R>* line = lds + c * p.LP;
+    cplx<R> v[K][RAD];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int j = t + k * p.TL;
+        if (live && j < nbl) {
+#pragma unroll
+            for (int i = 0; i < RAD; i++) v[k][i] = line[lds_pad(i * nbl + j)];
+        }
+    }
+    const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(p.stw) + p.twoff[s];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int j = t + k * p.TL;
+        if (live && j < nbl) {
+            if (nbl > 1 && s > 0) {
+                const cplx<R>* w = tw + j * (RAD - 1);
+#pragma unroll
+                for (int i = 1; i < RAD; i++) v[k][i] = cmul(v[k][i], w[i - 1]);
+            }
+            Dft<RAD, R>::run(v[k]);
+            store_butterfly<R, RAD>(p, v[k], j, nbl, dst, l0);
+        }
+    }
+}
+
+// One-stage lengths (L <= 16): load, DFT, store, no LDS.
+template <typename R, int E, int RAD>
+__device__ __forceinline__ void single_stage_direct(const PassDesc& p, int t, const cplx<R>* src, cplx<R>* dst,
+                                                    uint32_t l0, bool live)
+{
+    if (!live || t != 0) return;
+    cplx<R> v[RAD];
+#pragma unroll
+    for (int i = 0; i < RAD; i++) {
+        v[i] = {R(0), R(0)};
+        if (i < p.nvalid_in) v[i] = src[(int64_t)i * p.ies];
+        if (p.flags & kConjIn) v[i].y = -v[i].y;
+    }
+    Dft<RAD, R>::run(v);
+    store_butterfly<R, RAD>(p, v, 0, 1, dst, l0);
+}
+
+#define FFTZ_RADIX_SWITCH(r, ALLRAD, CALL)                 \
+    switch (r) {                                           \
+        case 2: { constexpr int RAD = 2; CALL; } break;    \
+        case 4: { constexpr int RAD = 4; CALL; } break;    \
+        case 8: { constexpr int RAD = 8; CALL; } break;    \
+        case 16: { constexpr int RAD = 16; CALL; } break;  \
+        default:                                           \
+            if constexpr (ALLRAD) {                        \
+                switch (r) {                               \
+                    case 3: { constexpr int RAD = 3; CALL; } break;   \
+                    case 5: { constexpr int RAD = 5; CALL; } break;   \
+                    case 6: { constexpr int RAD = 6; CALL; } break;   \
+                    case 7: { constexpr int RAD = 7; CALL; } break;   \
+                    case 10: { constexpr int RAD = 10; CALL; } break; \
+                    case 11: { constexpr int RAD = 11; CALL; } break; \
+                    case 12: { constexpr int RAD = 12; CALL; } break; \
+                    case 13: { constexpr int RAD = 13; CALL; } break; \
+                    default: break;                        \
+                }                                          \
+            }                                              \
+            break;                                         \
+    }
+
+// Complex-in pass with register-direct first / last stages.
+template <typename R, bool ALLRAD, int E>
+__device__ __forceinline__ void pass_body_direct(const PassDesc& p, cplx<R>* lds, int64_t l_base, int tid)
+{
+    const bool in_col = (p.flags & kInCol) != 0, out_col = (p.flags & kOutCol) != 0;
+    const int ci = in_col ? (tid & (p.C - 1)) : (tid >> p.tl_shift);
+    const int ti = in_col ? (tid >> p.c_shift) : (tid & (p.TL - 1));
+    const int co = out_col ? (tid & (p.C - 1)) : (tid >> p.tl_shift);
+    const int to = out_col ? (tid >> p.c_shift) : (tid & (p.TL - 1));
+    const bool live_in = ci < p.C && (l_base + ci) < p.nlines;
+    const bool live_out = co < p.C && (l_base + co) < p.nlines;
+    const cplx<R>* src = nullptr;
+    if (live_in) src = reinterpret_cast<const cplx<R>*>(p.in) + line_index(p, (uint32_t)(l_base + ci)).ioff;
+    cplx<R>* dst = nullptr;
+    uint32_t l0 = 0;
+    if (live_out) {
+        LineIdx li = line_index(p, (uint32_t)(l_base + co));
+        dst = reinterpret_cast<cplx<R>*>(p.out) + li.ooff;
+        l0 = li.l0;
+    }
+    const int S = p.nst;
+    if (S == 1) {
+        // the loading thread stores: use the input-side line for both
+        cplx<R>* d1 = nullptr;
+        uint32_t l01 = 0;
+        if (live_in) {
+            LineIdx li = line_index(p, (uint32_t)(l_base + ci));
+            d1 = reinterpret_cast<cplx<R>*>(p.out) + li.ooff;
+            l01 = li.l0;
+        }
+        FFTZ_RADIX_SWITCH(p.radix[0], ALLRAD, (single_stage_direct<R, E, RAD>(p, ti, src, d1, l01, live_in)))
+        return;
+    }
+    FFTZ_RADIX_SWITCH(p.radix[0], ALLRAD, (first_stage_direct<R, E, RAD>(p, lds, ci, ti, src, live_in)))
+    __syncthreads();
+    run_stages<R, ALLRAD, E>(p, lds, tid, 1, S - 1);
+    FFTZ_RADIX_SWITCH(p.radix[S - 1], ALLRAD,
+                      (last_stage_direct<R, E, RAD>(p, lds, co, to, dst, l0, live_out, S - 1)))
+}
+
 template <typename R, bool ALLRAD, int E>
 __device__ __forceinline__ void pass_body(const PassDesc& p, cplx<R>* lds, int64_t l_base, int tid)
 {
+    // (a zero-stage plan, L = 1, is a strided copy: the staged path below)
+    if (!(p.flags & (kRealIn | kC2RPre | kR2CPost)) && p.nst >= 1) {
+        pass_body_direct<R, ALLRAD, E>(p, lds, l_base, tid);
+        return;
+    }
     load_tile<R, E>(p, lds, l_base, tid);
     __syncthreads();
     if (p.flags & kC2RPre) {
         c2r_separate<R>(p, lds, tid);
         __syncthreads();
     }
-    run_stages<R, ALLRAD, E>(p, lds, tid);
+    run_stages<R, ALLRAD, E>(p, lds, tid, 0, p.nst);
     if (p.flags & kR2CPost) {
         r2c_recombine_store<R>(p, lds, l_base, tid);
     } else {
@@ -356,8 +585,10 @@ __device__ __forceinline__ void pass_body(const PassDesc& p, cplx<R>* lds, int64
 }
 
 template <typename R, bool ALLRAD, int E, int MAXT>
-__global__ void __launch_bounds__(MAXT) k_fft_pass(const PassDesc p)
+__global__ void __launch_bounds__(MAXT) k_fft_pass(const PassDesc p_arg)
 {
+    (void)p_arg;
+    const PassDesc& p = kernarg_desc();
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     cplx<R>* lds = reinterpret_cast<cplx<R>*>(smem_raw);
     const int tid = threadIdx.x;
@@ -404,7 +635,7 @@ __device__ __forceinline__ void bluestein_body(const PassDesc& p, cplx<R>* lds, 
         }
     }
     __syncthreads();
-    run_stages<R, ALLRAD, E>(p, lds, tid);
+    run_stages<R, ALLRAD, E>(p, lds, tid, 0, p.nst);
     if (c < p.C) {
         for (int e = t; e < M; e += p.TL) {
             cplx<R> a = line[lds_pad(e)];
@@ -412,7 +643,7 @@ __device__ __forceinline__ void bluestein_body(const PassDesc& p, cplx<R>* lds, 
         }
     }
     __syncthreads();
-    run_stages<R, ALLRAD, E>(p, lds, tid);
+    run_stages<R, ALLRAD, E>(p, lds, tid, 0, p.nst);
     if (live) {
         cplx<R>* dst = reinterpret_cast<cplx<R>*>(p.out) + li.ooff;
         for (int k = t; k < n; k += p.TL) {
@@ -425,12 +656,164 @@ __device__ __forceinline__ void bluestein_body(const PassDesc& p, cplx<R>* lds, 
     }
 }
 
-template <typename R, bool ALLRAD, int E, int MAXT>
-__global__ void __launch_bounds__(MAXT) k_bluestein(const PassDesc p)
+// Register-direct Bluestein stages (M-point plan with at least two stages):
+//   bs_first:  stage 0 of FFT_M(a) on operands a_e = x_e conj(B_e) (e < n,
+//              zero beyond), loaded straight from HBM;
+//   bs_spec:   last stage of FFT_M(a); each thread rewrites exactly the
+//              positions it read with conj(A_e) * spec_e (no barrier needed);
+//   bs_final:  last stage of the second FFT_M, X_k = conj(B_k W_k) (forward),
+//              stored straight to HBM for k < n.
+template <typename R, int E, int RAD>
+__device__ __forceinline__ void bs_first(const PassDesc& p, cplx<R>* lds, int c, int t, const cplx<R>* src, bool live)
 {
+    constexpr int K = (E + RAD - 1) / RAD;
+    const int nbl = p.L / RAD;
+    const int n = p.nvalid_in;
+    const cplx<R>* chirp = reinterpret_cast<const cplx<R>*>(p.chirp);
+    const bool cj = (p.flags & kConjIn) != 0;
+    cplx<R> v[K][RAD];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int j = t + k * p.TL;
+        if (live && j < nbl) {
+#pragma unroll
+            for (int i = 0; i < RAD; i++) {
+                const int e = i * nbl + j;
+                v[k][i] = {R(0), R(0)};
+                if (e < n) {
+                    cplx<R> x = src[(int64_t)e * p.ies];
+                    if (cj) x.y = -x.y;
+                    v[k][i] = cmul(x, chirp[e]);
+                }
+            }
+        }
+    }
+    cplx<R>* line = lds + c * p.LP;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int j = t + k * p.TL;
+        if (live && j < nbl) {
+            Dft<RAD, R>::run(v[k]);
+#pragma unroll
+            for (int i = 0; i < RAD; i++) line[lds_pad(j * RAD + i)] = v[k][i];
+        }
+    }
+}
+
+template <typename R, int E, int RAD>
+__device__ __forceinline__ void bs_last_common(const PassDesc& p, const cplx<R>* line, int t, bool live, int s,
+                                               cplx<R> (&v)[(E + RAD - 1) / RAD][RAD])
+{
+    constexpr int K = (E + RAD - 1) / RAD;
+    const int nbl = p.L / RAD;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int j = t + k * p.TL;
+        if (live && j < nbl) {
+#pragma unroll
+            for (int i = 0; i < RAD; i++) v[k][i] = line[lds_pad(i * nbl + j)];
+        }
+    }
+    const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(p.stw) + p.twoff[s];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int j = t + k * p.TL;
+        if (live && j < nbl) {
+            if (nbl > 1) {
+                const cplx<R>* w = tw + j * (RAD - 1);
+#pragma unroll
+                for (int i = 1; i < RAD; i++) v[k][i] = cmul(v[k][i], w[i - 1]);
+            }
+            Dft<RAD, R>::run(v[k]);
+        }
+    }
+}
+
+template <typename R, int E, int RAD>
+__device__ __forceinline__ void bs_spec(const PassDesc& p, cplx<R>* lds, int c, int t, bool live, int s)
+{
+    constexpr int K = (E + RAD - 1) / RAD;
+    const int nbl = p.L / RAD;
+    cplx<R>* line = lds + c * p.LP;
+    cplx<R> v[K][RAD];
+    bs_last_common<R, E, RAD>(p, line, t, live, s, v);
+    const cplx<R>* spec = reinterpret_cast<const cplx<R>*>(p.spec);
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int j = t + k * p.TL;
+        if (live && j < nbl) {
+#pragma unroll
+            for (int i = 0; i < RAD; i++) {
+                const int e = i * nbl + j;
+                line[lds_pad(e)] = cmul(cconj(v[k][i]), spec[e]);
+            }
+        }
+    }
+}
+
+template <typename R, int E, int RAD>
+__device__ __forceinline__ void bs_final(const PassDesc& p, const cplx<R>* lds, int c, int t, cplx<R>* dst, bool live,
+                                         int s)
+{
+    constexpr int K = (E + RAD - 1) / RAD;
+    const int nbl = p.L / RAD;
+    const int n = p.nvalid_out;
+    const cplx<R>* line = lds + c * p.LP;
+    cplx<R> v[K][RAD];
+    bs_last_common<R, E, RAD>(p, line, t, live, s, v);
+    const cplx<R>* chirp = reinterpret_cast<const cplx<R>*>(p.chirp);
+    const bool fwd = !(p.flags & kConjOut);
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int j = t + k * p.TL;
+        if (live && j < nbl) {
+#pragma unroll
+            for (int i = 0; i < RAD; i++) {
+                const int e = i * nbl + j;
+                if (e < n) {
+                    cplx<R> y = cmul(cconj(chirp[e]), v[k][i]);  // B * W
+                    if (fwd) y.y = -y.y;                          // forward result is conj(B W)
+                    dst[(int64_t)e * p.oes] = y;
+                }
+            }
+        }
+    }
+}
+
+template <typename R, bool ALLRAD, int E>
+__device__ __forceinline__ void bluestein_body_direct(const PassDesc& p, cplx<R>* lds, int64_t l_base, int tid)
+{
+    const int c = tid >> p.tl_shift;
+    const int t = tid & (p.TL - 1);
+    const bool live = c < p.C && (l_base + c) < p.nlines;
+    const cplx<R>* src = nullptr;
+    cplx<R>* dst = nullptr;
+    if (live) {
+        LineIdx li = line_index(p, (uint32_t)(l_base + c));
+        src = reinterpret_cast<const cplx<R>*>(p.in) + li.ioff;
+        dst = reinterpret_cast<cplx<R>*>(p.out) + li.ooff;
+    }
+    const int S = p.nst;
+    FFTZ_RADIX_SWITCH(p.radix[0], ALLRAD, (bs_first<R, E, RAD>(p, lds, c, t, src, live)))
+    __syncthreads();
+    run_stages<R, ALLRAD, E>(p, lds, tid, 1, S - 1);
+    FFTZ_RADIX_SWITCH(p.radix[S - 1], ALLRAD, (bs_spec<R, E, RAD>(p, lds, c, t, live, S - 1)))
+    __syncthreads();
+    run_stages<R, ALLRAD, E>(p, lds, tid, 0, S - 1);
+    FFTZ_RADIX_SWITCH(p.radix[S - 1], ALLRAD, (bs_final<R, E, RAD>(p, lds, c, t, dst, live, S - 1)))
+}
+
+template <typename R, bool ALLRAD, int E, int MAXT>
+__global__ void __launch_bounds__(MAXT) k_bluestein(const PassDesc p_arg)
+{
+    (void)p_arg;
+    const PassDesc& p = kernarg_desc();
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     cplx<R>* lds = reinterpret_cast<cplx<R>*>(smem_raw);
-    bluestein_body<R, ALLRAD, E>(p, lds, (int64_t)blockIdx.x * p.C, threadIdx.x);
+    if (p.nst >= 2)
+        bluestein_body_direct<R, ALLRAD, E>(p, lds, (int64_t)blockIdx.x * p.C, threadIdx.x);
+    else
+        bluestein_body<R, ALLRAD, E>(p, lds, (int64_t)blockIdx.x * p.C, threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -478,7 +861,10 @@ __global__ void __launch_bounds__(MAXT) k_batch(const BatchArgs a)
         }
         __syncthreads();
         if (bt.bluestein) {
-            bluestein_body<R, true, E>(pd, lds, 0, tid);
+            if (pd.nst >= 2)
+                bluestein_body_direct<R, true, E>(pd, lds, 0, tid);
+            else
+                bluestein_body<R, true, E>(pd, lds, 0, tid);
         } else {
             pass_body<R, true, E>(pd, lds, 0, tid);
         }

@@ -1,0 +1,341 @@
+// Split-precision matrix-core pass for power-of-16 lengths (L = 256, 4096),
+// complex64 data.
+//
+// Every radix-16 Stockham stage is a small DFT-matrix GEMM on the MFMA units:
+// for a block of 16 butterflies (columns) with inputs X (16 complex values
+// each),
+//   [Yr]   [Fr  -Fi] [Xr]
+//   [Yi] = [Fi   Fr] [Xi],    F[k][j] = exp(-2 pi i j k / 16),
+// a real 32x32 by 32x16 product: two v_mfma_f32_16x16x32 per product term
+// (rows 0-15 give Yr, rows 16-31 give Yi), K = 32 = [Xr; Xi].
+//
+// Operand precision (MODE):
+//   kMfmaF32   -- plain FP32 operands on v_mfma_f32_16x16x4_f32 (8 K-steps);
+//   kMfmaBf16x3 -- F and X split into bf16 hi + lo, three products
+//                  (Fl Xh + Fh Xl + Fh Xh), FP32 accumulate;
+//   kMfmaF16x2  -- X scaled per column by a power of two so its largest entry
+//                  is below 2^14, F and X split into fp16 hi + lo, the same
+//                  three products, FP32 accumulate, scale undone exactly.
+// The DFT-matrix fragments (already split, exact FP64 products rounded once)
+// come from a per-lane table built on the host.
+//
+// Data path per tile (16 column blocks, 4 per wavefront):
+//   stage 0 gathers its operands straight from HBM;
+//   every stage's result is multiplied by the next stage's twiddle (FP32
+//   stage tables, w_{16 ls'}^(i' c')) and written to LDS;
+//   the last stage stores straight to HBM.
+// Hardware operand layout of the 16x16x32 forms: lane l holds A[l % 16][8 (l/16) + t]
+// and B[8 (l/16) + t][l % 16], t < 8; result D[4 (l/16) + r][l % 16], r < 4.
+// The 16x16x4 FP32 form: A[l % 16][l / 16], B[l / 16][l % 16], same D.  (Both
+// checked on gfx950 with exact small-integer operands.)
+#include <hip/hip_runtime.h>
+
+#include "codelets.h"
+#include "kernels.h"
+#include "pass.h"
+
+namespace fftz {
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void mfma_barrier()
+{
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+__device__ __forceinline__ uint32_t fdiv_m(uint32_t n, const FastDiv& f)
+{
+    return (__umulhi(n, f.mul) + n) >> f.shift;
+}
+
+struct LineOffM {
+    int64_t ioff, ooff;
+};
+
+__device__ __forceinline__ LineOffM line_off_m(const PassDesc& p, uint32_t l)
+{
+    uint32_t r = fdiv_m(l, p.ln0);
+    uint32_t l0 = l - r * p.ln0.d;
+    uint32_t l2 = fdiv_m(r, p.ln1);
+    uint32_t l1 = r - l2 * p.ln1.d;
+    LineOffM o;
+    o.ioff = (int64_t)l0 * p.ils[0] + (int64_t)l1 * p.ils[1] + (int64_t)l2 * p.ils[2];
+    o.ooff = (int64_t)l0 * p.ols[0] + (int64_t)l1 * p.ols[1] + (int64_t)l2 * p.ols[2];
+    return o;
+}
+
+__device__ __forceinline__ int padm(int e) { return e + (e >> 4); }
+
+// Operand layout (all modes).  The GEMM's K index is permuted so every lane
+// holds four whole complex inputs of its column: lane group g = lane / 16
+// holds rows rho(g, q) = 4 q + g, q < 4, as the 8-vector
+//   t = 2 q + part:  part 0 -> Re x[rho], part 1 -> Im x[rho],
+// i.e. the natural (re, im) memory order.  The host builds the DFT-matrix
+// fragments in the same K order (A_top[t] = Fr or -Fi, A_bot[t] = Fi or Fr).
+// For the FP32 form the 8-vector is consumed one element per K-step.
+__device__ __forceinline__ int rho(int g, int q) { return 4 * q + g; }
+
+template <int MODE>
+struct Frag;
+template <>
+struct Frag<kMfmaF32> {
+    float top[8], bot[8];
+    __device__ __forceinline__ void init(const float* a)
+    {
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+            top[t] = a[t];
+            bot[t] = a[8 + t];
+        }
+    }
+};
+template <>
+struct Frag<kMfmaBf16x3> {
+    bf16x8 th, tl, bh, bl;
+    __device__ __forceinline__ void init(const float* a)
+    {
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+            th[t] = (__bf16)a[t];
+            tl[t] = (__bf16)a[8 + t];
+            bh[t] = (__bf16)a[16 + t];
+            bl[t] = (__bf16)a[24 + t];
+        }
+    }
+};
+template <>
+struct Frag<kMfmaF16x2> {
+    f16x8 th, tl, bh, bl;
+    __device__ __forceinline__ void init(const float* a)
+    {
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+            th[t] = (_Float16)a[t];
+            tl[t] = (_Float16)a[8 + t];
+            bh[t] = (_Float16)a[16 + t];
+            bl[t] = (_Float16)a[24 + t];
+        }
+    }
+};
+
+// Y = F X for NB column blocks at once (independent MFMA chains interleaved);
+// yr[b][r], yi[b][r] = Re, Im of Y[4 g + r] of block b.
+template <int MODE, int NB>
+__device__ __forceinline__ void dft16_blocks(const float (&x)[NB][8], const Frag<MODE>& A, f32x4 (&yr)[NB],
+                                             f32x4 (&yi)[NB])
+{
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+        yr[b] = f32x4{0.f, 0.f, 0.f, 0.f};
+        yi[b] = f32x4{0.f, 0.f, 0.f, 0.f};
+    }
+    if constexpr (MODE == kMfmaF32) {
+#pragma unroll
+        for (int kk = 0; kk < 8; kk++)
+#pragma unroll
+            for (int b = 0; b < NB; b++) {
+                yr[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(A.top[kk], x[b][kk], yr[b], 0, 0, 0);
+                yi[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(A.bot[kk], x[b][kk], yi[b], 0, 0, 0);
+            }
+    } else if constexpr (MODE == kMfmaBf16x3) {
+        bf16x8 xh[NB], xl[NB];
+#pragma unroll
+        for (int b = 0; b < NB; b++)
+#pragma unroll
+            for (int t = 0; t < 8; t++) {
+                const __bf16 h = (__bf16)x[b][t];
+                xh[b][t] = h;
+                xl[b][t] = (__bf16)(x[b][t] - (float)h);
+            }
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+            yr[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A.tl, xh[b], yr[b], 0, 0, 0);
+            yi[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A.bl, xh[b], yi[b], 0, 0, 0);
+        }
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+            yr[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A.th, xl[b], yr[b], 0, 0, 0);
+            yi[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A.bh, xl[b], yi[b], 0, 0, 0);
+        }
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+            yr[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A.th, xh[b], yr[b], 0, 0, 0);
+            yi[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A.bh, xh[b], yi[b], 0, 0, 0);
+        }
+    } else {
+        // per-column power-of-two scale: a column's 32 values sit in lanes
+        // l, l^16, l^32, l^48
+        f16x8 xh[NB], xl[NB];
+        float inv[NB];
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+            float m = 0.f;
+#pragma unroll
+            for (int t = 0; t < 8; t++) m = fmaxf(m, fabsf(x[b][t]));
+            m = fmaxf(m, __shfl_xor(m, 16));
+            m = fmaxf(m, __shfl_xor(m, 32));
+            int e = 0;
+            (void)frexpf(m, &e);  // m < 2^e
+            e = e < -100 ? -100 : e;
+            const float sc = ldexpf(1.0f, 14 - e);
+            inv[b] = ldexpf(1.0f, e - 14);
+#pragma unroll
+            for (int t = 0; t < 8; t++) {
+                const float v = x[b][t] * sc;
+                const _Float16 h = (_Float16)v;
+                xh[b][t] = h;
+                xl[b][t] = (_Float16)(v - (float)h);
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+            yr[b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A.tl, xh[b], yr[b], 0, 0, 0);
+            yi[b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A.bl, xh[b], yi[b], 0, 0, 0);
+        }
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+            yr[b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A.th, xl[b], yr[b], 0, 0, 0);
+            yi[b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A.bh, xl[b], yi[b], 0, 0, 0);
+        }
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+            yr[b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A.th, xh[b], yr[b], 0, 0, 0);
+            yi[b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A.bh, xh[b], yi[b], 0, 0, 0);
+        }
+#pragma unroll
+        for (int b = 0; b < NB; b++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                yr[b][r] *= inv[b];
+                yi[b][r] *= inv[b];
+            }
+    }
+}
+
+}  // namespace
+
+// One workgroup (4 wavefronts) per tile of C lines, C * L / 256 = 16 column
+// blocks per stage, 4 per wavefront.
+template <int MODE, int L>
+__global__ void __launch_bounds__(256) k_mfma_pass(const PassDesc p, const float* __restrict__ frag)
+{
+    constexpr int S = L == 256 ? 2 : 3;
+    constexpr int nbl = L / 16;           // butterflies per line per stage
+    constexpr int CPL = L / 256;          // column blocks per line
+    constexpr int LP = ((L - 1) + ((L - 1) >> 4) + 1) | 1;
+    constexpr int NB = 4;                 // column blocks per wavefront
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+    cplx<float>* lds = reinterpret_cast<cplx<float>*>(smem_raw);
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wv = tid >> 6;
+    const int cl = lane & 15, g = lane >> 4;
+    const int64_t l_base = (int64_t)blockIdx.x * p.C;
+
+    Frag<MODE> A;
+    {
+        float af[32];
+        const float4* fp = reinterpret_cast<const float4*>(frag + lane * 32);
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const float4 v = fp[q];
+            af[4 * q] = v.x;
+            af[4 * q + 1] = v.y;
+            af[4 * q + 2] = v.z;
+            af[4 * q + 3] = v.w;
+        }
+        A.init(af);
+    }
+    const float cj = (p.flags & kConjIn) ? -1.0f : 1.0f;
+    const cplx<float>* twb = reinterpret_cast<const cplx<float>*>(p.stw);
+
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        const int ls = s == 0 ? 1 : (s == 1 ? 16 : 256);
+        float x[NB][8];
+        // gather: four complex inputs per lane per block
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+            const int blk = wv * NB + b;
+            const int c = blk / CPL;
+            const int j = (blk % CPL) * 16 + cl;
+            cplx<float> v[4];
+            if (s == 0) {
+                if ((l_base + c) < p.nlines) {
+                    const int64_t ioff = line_off_m(p, (uint32_t)(l_base + c)).ioff;
+                    const cplx<float>* src = reinterpret_cast<const cplx<float>*>(p.in) + ioff + (int64_t)j * p.ies;
+                    const int64_t rstep = (int64_t)nbl * p.ies;
+#pragma unroll
+                    for (int q = 0; q < 4; q++) v[q] = src[rho(g, q) * rstep];
+                } else {
+#pragma unroll
+                    for (int q = 0; q < 4; q++) v[q] = {0.f, 0.f};
+                }
+#pragma unroll
+                for (int q = 0; q < 4; q++) v[q].y *= cj;
+            } else {
+                const cplx<float>* line = lds + c * LP;
+#pragma unroll
+                for (int q = 0; q < 4; q++) v[q] = line[padm(rho(g, q) * nbl + j)];
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                x[b][2 * q] = v[q].x;
+                x[b][2 * q + 1] = v[q].y;
+            }
+        }
+        if (s > 0) mfma_barrier();  // every read of this stage done before its writes
+        f32x4 yr[NB], yi[NB];
+        dft16_blocks<MODE, NB>(x, A, yr, yi);
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+            const int blk = wv * NB + b;
+            const int c = blk / CPL;
+            const int j = (blk % CPL) * 16 + cl;
+            const int gg = j / ls, cc = j % ls;
+            if (s < S - 1) {
+                cplx<float>* line = lds + c * LP;
+                const int ls2 = ls * 16;
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int k = 4 * g + r;
+                    const int pos = gg * 16 * ls + k * ls + cc;
+                    // next stage: pos = i2 * nbl + j2, twiddle w_{16 ls2}^(i2 * (j2 % ls2))
+                    const int i2 = pos / nbl, j2 = pos % nbl, c2 = j2 % ls2;
+                    cplx<float> y = {yr[b][r], yi[b][r]};
+                    if (i2 > 0 && c2 > 0) y = cmul(y, twb[p.twoff[s + 1] + c2 * 15 + (i2 - 1)]);
+                    line[padm(pos)] = y;
+                }
+            } else if ((l_base + c) < p.nlines) {
+                const int64_t ooff = line_off_m(p, (uint32_t)(l_base + c)).ooff;
+                cplx<float>* dst = reinterpret_cast<cplx<float>*>(p.out) + ooff + (int64_t)j * p.oes;
+                const int64_t rstep = (int64_t)nbl * p.oes;
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    cplx<float> y = {yr[b][r], yi[b][r]};
+                    if (p.flags & kScale) y = cscale(y, p.scale_f);
+                    if (p.flags & kConjOut) y.y = -y.y;
+                    dst[(4 * g + r) * rstep] = y;  // last stage: gg = 0, cc = j
+                }
+            }
+        }
+        if (s < S - 1) mfma_barrier();
+    }
+}
+
+const void* kernel_symbol_mfma(int mode, int L)
+{
+    switch (mode) {
+        case kMfmaF32: return L == 256 ? (const void*)&k_mfma_pass<kMfmaF32, 256> : (const void*)&k_mfma_pass<kMfmaF32, 4096>;
+        case kMfmaBf16x3:
+            return L == 256 ? (const void*)&k_mfma_pass<kMfmaBf16x3, 256> : (const void*)&k_mfma_pass<kMfmaBf16x3, 4096>;
+        case kMfmaF16x2:
+            return L == 256 ? (const void*)&k_mfma_pass<kMfmaF16x2, 256> : (const void*)&k_mfma_pass<kMfmaF16x2, 4096>;
+        default: return nullptr;
+    }
+}
+
+}  // namespace fftz

@@ -38,6 +38,14 @@ enum PassFlags : uint32_t {
     kRealOut = 1u << 10,   // C2R: output is real, written as L complex pairs
 };
 
+// Operand precision of the matrix-core radix-16 pass (kernels_mfma.hip).
+enum MfmaMode : int {
+    kMfmaNone = 0,    // plain FP32 / FP64 Stockham codelets on the vector ALUs
+    kMfmaF32 = 1,     // DFT-16 GEMM with FP32 operands (v_mfma_f32_16x16x4_f32)
+    kMfmaBf16x3 = 2,  // bf16 hi + lo split, three products, FP32 accumulate
+    kMfmaF16x2 = 3,   // per-column power-of-two scale, fp16 hi + lo, three products
+};
+
 // n / d and n % d for 0 <= n < 2^31 with one mul_hi (Granlund-Montgomery).
 struct FastDiv {
     uint32_t mul;

@@ -59,6 +59,10 @@ const DeviceInfo& device_info()
             for (int L = 64; L <= 4096; L *= 2)
                 (void)hipFuncSetAttribute(kernel_symbol_fixed(prec, L), hipFuncAttributeMaxDynamicSharedMemorySize,
                                           (int)info.lds_per_block);
+        for (int m : {kMfmaF32, kMfmaBf16x3, kMfmaF16x2})
+            for (int L : {256, 4096})
+                (void)hipFuncSetAttribute(kernel_symbol_mfma(m, L), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          (int)info.lds_per_block);
     });
     return info;
 }
@@ -296,6 +300,63 @@ static std::vector<LineAxis> merge_axes(std::vector<LineAxis> ax)
     return ax;
 }
 
+// Round x to `bits` significant bits, round-to-nearest-even, with a floor
+// on the quantum exponent (subnormal spacing): bf16 = (8, -133), fp16 = (11, -24).
+static double round_sig(double x, int bits, int qmin)
+{
+    if (x == 0.0) return 0.0;
+    int e = 0;
+    (void)std::frexp(x, &e);  // |x| in [2^(e-1), 2^e)
+    const int q = std::max(e - bits, qmin);
+    return std::ldexp(std::nearbyint(std::ldexp(x, -q)), q);
+}
+
+// Per-lane DFT-16 operand fragments for the matrix-core pass.  The real
+// 32 x 32 operator is A = [[Fr, -Fi], [Fi, Fr]] with F[k][j] = w_16^(jk)
+// (forward); rows 0-15 ("top") produce Re Y, rows 16-31 ("bottom") Im Y;
+// column k < 16 multiplies Re x[k], column 16 + k multiplies Im x[k].
+// The kernel permutes K so that lane group g = l / 16 supplies, as its
+// element t < 8, part t & 1 (re / im) of input row rho = 4 (t >> 1) + g.
+// Lane l (output row r = l % 16), 32 floats:
+//   kMfmaF32:  [0..7] top[r][K(g, t)], [8..15] bottom[r][K(g, t)];
+//   split:     [0..7] hi(top), [8..15] lo(top), [16..23] hi(bottom),
+//              [24..31] lo(bottom); hi = round(a), lo = round(a - hi) in bf16
+//              (kMfmaBf16x3) or fp16 (kMfmaF16x2), stored as the exactly
+//              representable float values.
+static int64_t mfma_fragments(TableBuilder& tb, int mode)
+{
+    std::vector<float> f(64 * 32, 0.0f);
+    auto A = [](int half, int r, int k) -> double {
+        const d2 w = root_fwd((int64_t)r * (k & 15), 16);  // (Fr, Fi)
+        const bool im = k >= 16;
+        if (half == 0) return im ? -w.y : w.x;
+        return im ? w.x : w.y;
+    };
+    const int bits = mode == kMfmaBf16x3 ? 8 : 11;
+    const int qmin = mode == kMfmaBf16x3 ? -133 : -24;
+    for (int l = 0; l < 64; l++) {
+        const int r = l & 15, g = l >> 4;
+        float* o = &f[(size_t)l * 32];
+        auto kidx = [g](int t) { return 16 * (t & 1) + 4 * (t >> 1) + g; };
+        if (mode == kMfmaF32) {
+            for (int t = 0; t < 8; t++) {
+                o[t] = (float)A(0, r, kidx(t));
+                o[8 + t] = (float)A(1, r, kidx(t));
+            }
+            continue;
+        }
+        for (int t = 0; t < 8; t++)
+            for (int half = 0; half < 2; half++) {
+                const double a = A(half, r, kidx(t));
+                const double hi = round_sig(a, bits, qmin);
+                const double lo = round_sig(a - hi, bits, qmin);
+                o[16 * half + t] = (float)hi;
+                o[16 * half + 8 + t] = (float)lo;
+            }
+    }
+    return tb.add(f.data(), f.size() * sizeof(float));
+}
+
 static bool build_pass(Plan* plan, TableBuilder& tb, const PassReq& q, std::string& err)
 {
     const int prec = plan->spec.prec_bytes;
@@ -331,6 +392,10 @@ static bool build_pass(Plan* plan, TableBuilder& tb, const PassReq& q, std::stri
     // radix variant (fewer live VGPRs) unless that needs > 512 threads/line
     int E = 16;
     if (ln.all_radix && next_pow2((q.L + 7) / 8) <= 512) E = 8;
+    // complex128 outside the fixed-kernel range: 8 values per thread keeps
+    // the 1024-thread variant within its 128 VGPRs (16 would spill)
+    const bool fixed_range = q.L >= 64 && q.L <= 4096 && (q.L & (q.L - 1)) == 0;
+    if (prec == 8 && !fixed_range && next_pow2((q.L + 7) / 8) <= 1024) E = 8;
     if (env_int("FFTZ_E", 0) == 8 || env_int("FFTZ_E", 0) == 16) {
         const int e = env_int("FFTZ_E", 0);
         if (next_pow2((q.L + e - 1) / e) <= 1024) E = e;
@@ -360,6 +425,11 @@ static bool build_pass(Plan* plan, TableBuilder& tb, const PassReq& q, std::stri
             plan_match = radix[s] == (s < S - 1 ? 16 : (1 << rem));
     }
     const int tmax = plan_match ? 256 : 1024;
+    // matrix-core pass: complex64, L = 16^2 or 16^3 with the plain 16-radix
+    // plan, row mapping, no fused twiddle / real-packed step
+    const int mm = plan->spec.mfma_mode;
+    const bool use_mfma = mm != kMfmaNone && prec == 4 && plan_match && (q.L == 256 || q.L == 4096) &&
+                          !q.want_in_col && !q.want_out_col && !(q.flags & kTw4);
 
     // line axes: choose l0
     std::vector<LineAxis> ax = merge_axes(q.lines);
@@ -392,7 +462,13 @@ static bool build_pass(Plan* plan, TableBuilder& tb, const PassReq& q, std::stri
         return false;
     }
     int C = 1;
-    if (in_col || out_col) {
+    if (use_mfma) {
+        in_col = out_col = false;
+        C = q.L == 4096 ? 1 : 16;  // 16 column blocks of 16 butterflies per tile
+        TL = 16;
+        d.TL = TL;
+        d.tl_shift = 4;
+    } else if (in_col || out_col) {
         const int cmax = env_int("FFTZ_COL_C", prec == 8 ? 16 : 32);
         const size_t budget = std::min(lds_cap, (size_t)env_int("FFTZ_COL_LDS", 96 * 1024));
         while (C * 2 <= cmax && (size_t)(C * 2) * line_bytes <= budget && (n0 % (C * 2)) == 0 &&
@@ -400,7 +476,7 @@ static bool build_pass(Plan* plan, TableBuilder& tb, const PassReq& q, std::stri
             C *= 2;
         if (n0 % C != 0 || C < 2) { in_col = false; out_col = false; C = 1; }
     }
-    if (!(in_col || out_col)) {
+    if (!(in_col || out_col) && !use_mfma) {
         const int tgt = env_int("FFTZ_ROW_T", 256);
         const size_t budget = std::min(lds_cap, (size_t)env_int("FFTZ_ROW_LDS", 64 * 1024));
         int64_t nl_total = 1;
@@ -446,7 +522,12 @@ static bool build_pass(Plan* plan, TableBuilder& tb, const PassReq& q, std::stri
     ln.out_role = q.out_role;
     ln.block = (uint32_t)(C * TL);
     ln.maxt = ln.block <= 256 ? 256 : 1024;
-    ln.fixed = plan_match && ln.block <= 256;
+    ln.fixed = plan_match && ln.block <= 256 && !use_mfma;
+    if (use_mfma) {
+        ln.mfma = mm;
+        ln.block = 256;
+        ln.frag_off = mfma_fragments(tb, mm);
+    }
     const int64_t tiles = (nl + C - 1) / C;
     ln.grid = (uint32_t)tiles;
     ln.lds_bytes = (uint32_t)((size_t)C * line_bytes);
@@ -1110,6 +1191,7 @@ Plan* plan_create(const ProblemSpec& spec_in, std::string& err, bool dry_run)
         if (ln.chirp_off >= 0) ln.d.chirp = base + ln.chirp_off;
         if (ln.spec_off >= 0) ln.d.spec = base + ln.spec_off;
         if (ln.rtw_off >= 0) ln.d.rtw = base + ln.rtw_off;
+        if (ln.frag_off >= 0) ln.frag = reinterpret_cast<const float*>(base + ln.frag_off);
     }
     plan->host_tables.clear();
     plan->host_tables.shrink_to_fit();
@@ -1168,7 +1250,7 @@ std::string plan_json(const Plan* p)
            << "," << d.ils[2] << "],\"ols\":[" << d.ols[0] << "," << d.ols[1] << "," << d.ols[2]
            << "],\"nlines\":" << d.nlines << ",\"ln0\":" << d.ln0.d << ",\"ln1\":" << d.ln1.d
            << ",\"flags\":" << d.flags << ",\"C\":" << d.C << ",\"TL\":" << d.TL << ",\"LP\":" << d.LP
-           << ",\"E\":" << ln.E << ",\"fixed\":" << (ln.fixed ? 1 : 0) << ",\"grid\":" << ln.grid << ",\"block\":" << ln.block
+           << ",\"E\":" << ln.E << ",\"fixed\":" << (ln.fixed ? 1 : 0) << ",\"mfma\":" << ln.mfma << ",\"grid\":" << ln.grid << ",\"block\":" << ln.block
            << ",\"lds\":" << ln.lds_bytes << ",\"in_role\":" << (int)ln.in_role
            << ",\"out_role\":" << (int)ln.out_role << ",\"in_off\":" << ln.in_off_bytes
            << ",\"out_off\":" << ln.out_off_bytes << ",\"tw4_n\":" << d.tw4_n << ",\"radix\":[";
@@ -1305,9 +1387,11 @@ static hipError_t launch_all(Plan* p, Slot* s, const void* in, void* out, hipStr
         };
         d.in = role_ptr(ln.in_role) + ln.in_off_bytes;
         d.out = role_ptr(ln.out_role) + ln.out_off_bytes;
-        void* args[] = {&d};
-        const void* fn = ln.fixed ? kernel_symbol_fixed(p->spec.prec_bytes, ln.d.L)
-                                  : kernel_symbol(ln.kind, p->spec.prec_bytes, ln.all_radix, ln.E, ln.maxt);
+        const float* frag = ln.frag;
+        void* args[] = {&d, &frag};
+        const void* fn = ln.mfma ? kernel_symbol_mfma(ln.mfma, ln.d.L)
+                         : ln.fixed ? kernel_symbol_fixed(p->spec.prec_bytes, ln.d.L)
+                                    : kernel_symbol(ln.kind, p->spec.prec_bytes, ln.all_radix, ln.E, ln.maxt);
         hipError_t e = hipLaunchKernel(fn, dim3(ln.grid),
                                        dim3(ln.block), args, ln.lds_bytes, stream);
         if (e != hipSuccess) return e;

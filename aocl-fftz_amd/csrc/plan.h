@@ -33,6 +33,7 @@ struct ProblemSpec {
     bool inplace = false;
     std::vector<AxisSpec> dims;  // dims[0] innermost
     std::vector<AxisSpec> vecs;  // batch axes
+    int mfma_mode = 0;           // MfmaMode for complex64 power-of-16 passes
 };
 
 // One uneven-batch entry: `batch` transforms of length n, `dist` elements
@@ -56,12 +57,15 @@ struct Launch {
     int E = 16;        // complex values per thread per stage
     int maxt = 256;    // block-size bound of the kernel variant
     bool fixed = false;  // fixed-length kernel (kernels_fixed.hip)
+    int mfma = 0;        // MfmaMode of a matrix-core pass (kernels_mfma.hip)
     PassDesc d{};
     Role in_role = Role::In, out_role = Role::Out;
     int64_t in_off_bytes = 0, out_off_bytes = 0;
     uint32_t grid = 0, block = 0, lds_bytes = 0;
     // table offsets in the plan arena (-1: none), resolved after upload
     int64_t stw_off = -1, tw4_off = -1, chirp_off = -1, spec_off = -1, rtw_off = -1;
+    int64_t frag_off = -1;  // MFMA DFT-16 operand fragments (64 lanes x 32 floats)
+    const float* frag = nullptr;
 };
 
 // Per-call execution context: a stream, the work buffer and host staging

@@ -66,6 +66,7 @@ EXPORTS = (
     "aoclfftz_setup_f", "aoclfftz_setup_d", "aoclfftz_setup_f_64_", "aoclfftz_setup_d_64_",
     "aoclfftz_execute", "aoclfftz_execute_io", "aoclfftz_destroy", "aoclfftz_version",
     "aoclfftz_gpu_set_stream", "aoclfftz_gpu_synchronize", "aoclfftz_gpu_set_graph",
+    "aoclfftz_gpu_set_compute_mode",
     "aoclfftz_gpu_plan_info", "aoclfftz_gpu_plan_flops", "aoclfftz_gpu_plan_device_bytes",
     "aoclfftz_gpu_plan_json", "aoclfftz_gpu_profile", "aoclfftz_gpu_profile_read",
     "aoclfftz_gpu_plan_launches", "aoclfftz_gpu_execute_many", "aoclfftz_gpu_batch_setup",
@@ -107,6 +108,8 @@ def load():
     lib.aoclfftz_gpu_synchronize.restype = ctypes.c_int
     lib.aoclfftz_gpu_set_graph.argtypes = [vp, ctypes.c_int32]
     lib.aoclfftz_gpu_set_graph.restype = ctypes.c_int
+    lib.aoclfftz_gpu_set_compute_mode.argtypes = [vp, ctypes.c_int32]
+    lib.aoclfftz_gpu_set_compute_mode.restype = ctypes.c_int
     lib.aoclfftz_gpu_plan_info.argtypes = [vp]
     lib.aoclfftz_gpu_plan_info.restype = ctypes.c_char_p
     lib.aoclfftz_gpu_plan_flops.argtypes = [vp]
@@ -216,6 +219,12 @@ class Plan:
 
     def set_graph(self, enable: bool) -> int:
         return load().aoclfftz_gpu_set_graph(self.handle, 1 if enable else 0)
+
+    # operand precision of complex64 radix-16 passes (aoclfftz_gpu_set_compute_mode)
+    PLAIN, MFMA_F32, MFMA_BF16X3, MFMA_F16X2 = 0, 1, 2, 3
+
+    def set_compute_mode(self, mode: int) -> int:
+        return load().aoclfftz_gpu_set_compute_mode(self.handle, int(mode))
 
     def profile(self, enable: bool) -> int:
         return load().aoclfftz_gpu_profile(self.handle, 1 if enable else 0)

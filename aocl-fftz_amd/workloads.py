@@ -55,3 +55,20 @@ def uneven_batch(count: int = 20000, seed: int = 3, max_batch: int = 64, smooth_
 
 def uneven_flops(items) -> float:
     return float(sum(5.0 * n * np.log2(n) * b for n, b, *_ in items))
+
+
+def sinusoids(n: int, batch: int, seed: int = 4, tones: int = 8, noise: float = 1e-3):
+    """Complex64 lines of length n: each a sum of `tones` complex exponentials
+    with random integer frequencies, amplitudes in [0.5, 2) and phases, plus
+    noise * normal(0, 1) in both parts."""
+    rng = np.random.default_rng(seed)
+    j = np.arange(n)
+    out = np.empty((batch, n), dtype=np.complex64)
+    for b in range(batch):
+        f = rng.integers(0, n, tones)
+        a = rng.uniform(0.5, 2.0, tones)
+        ph = rng.uniform(0.0, 2 * np.pi, tones)
+        x = (a[:, None] * np.exp(1j * (2 * np.pi * np.outer(f, j) / n + ph[:, None]))).sum(axis=0)
+        x = x + noise * (rng.standard_normal(n) + 1j * rng.standard_normal(n))
+        out[b] = x.astype(np.complex64)
+    return out

@@ -14,6 +14,9 @@ Also measured in the same run and reported in the same JSON line:
     not serve them, plus the cache-warm figure;
   * the uneven batch: 20,000 complex64 items of mixed length (7-smooth and
     prime) with batch 1..64 each, in one persistent launch per tile class;
+  * the precision workload: batched complex64 N = 4096, batch 16384, lines
+    of 8 tones + noise, plain FP32 against the matrix-core forms (FP32, bf16
+    hi+lo, scaled fp16 hi+lo operands), time and error against complex128;
   * roofline of the dominant kernel: algorithmic bytes per launch over its
     average duration measured with HIP events on its own stream;
   * the reference's CPU path (AOCL-FFTZ built from its sources, OpenMP) timed
@@ -271,6 +274,77 @@ def bench_uneven(torch, fftz, steps, warmup):
     return res
 
 
+def sinusoid_lines(torch, n, b, seed=4, tones=8, noise=1e-3, chunk=512):
+    """complex64 (b, n): per line, 8 complex tones with random integer
+    frequencies, amplitudes in [0.5, 2) and phases, plus noise*normal(0,1)."""
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    x = torch.empty((b, n), dtype=torch.complex64, device="cuda")
+    j = torch.arange(n, device="cuda", dtype=torch.float64)
+    for s in range(0, b, chunk):
+        m = min(chunk, b - s)
+        f = torch.randint(0, n, (m, tones, 1), device="cuda", generator=g).double()
+        a = torch.rand((m, tones, 1), device="cuda", generator=g, dtype=torch.float64) * 1.5 + 0.5
+        ph = torch.rand((m, tones, 1), device="cuda", generator=g, dtype=torch.float64) * (2 * math.pi)
+        ang = (2 * math.pi / n) * f * j + ph
+        v = torch.polar(a.expand_as(ang), ang).sum(dim=1)
+        v += noise * torch.complex(torch.randn((m, n), device="cuda", generator=g, dtype=torch.float64),
+                                   torch.randn((m, n), device="cuda", generator=g, dtype=torch.float64))
+        x[s:s + m] = v.to(torch.complex64)
+        del ang, v
+    return x
+
+
+def bench_split(torch, fftz, steps, warmup, n=4096, b=16384):
+    """Precision workload: batched complex64 C2C, N=4096, batch 16384, lines of
+    8 tones + 1e-3 noise; the plain FP32 pass against the matrix-core forms
+    (FP32 operands, bf16 hi+lo, scaled fp16 hi+lo).  Error of each against the
+    library's own complex128 transform of the same (upcast) input."""
+    x = sinusoid_lines(torch, n, b)
+    xd = x.to(torch.complex128)
+    yd = torch.empty_like(xd)
+    stream = torch.cuda.current_stream()
+    pd = fftz.Plan([(n, 1, 1)], [(b, n, n)], "d", 0)
+    pd.set_stream(stream)
+    assert pd.execute_io(xd, yd) == 0
+    torch.cuda.synchronize()
+    pd.destroy()
+    del xd
+    y = torch.empty_like(x)
+    flops = 5.0 * n * math.log2(n) * b
+    byts = 2 * 8.0 * n * b
+    names = {0: "plain_fp32", 1: "mfma_fp32", 2: "mfma_bf16x3", 3: "mfma_fp16x2"}
+    out = {}
+    ydn = torch.linalg.vector_norm(yd).item()
+    for mode in (0, 1, 2, 3):
+        p = fftz.Plan([(n, 1, 1)], [(b, n, n)], "f", 0)
+        if mode:
+            assert p.set_compute_mode(mode) == 0
+        p.set_stream(stream)
+        for _ in range(max(warmup, 1)):
+            p.execute_io(x, y)
+        torch.cuda.synchronize()
+        t0 = torch.cuda.Event(enable_timing=True)
+        t1 = torch.cuda.Event(enable_timing=True)
+        t0.record(stream)
+        for _ in range(steps):
+            assert p.execute_io(x, y) == 0
+        t1.record(stream)
+        torch.cuda.synchronize()
+        ms = t0.elapsed_time(t1) / steps
+        err = 0.0
+        for s in range(0, b, 2048):
+            err += torch.linalg.vector_norm(y[s:s + 2048].to(torch.complex128) - yd[s:s + 2048]).item() ** 2
+        out[names[mode]] = {"ms": round(ms, 4), "gflops": round(flops / (ms * 1e-3) / 1e9, 1),
+                            "frac_of_hbm": round(byts / (ms * 1e-3) / HBM_PEAK, 4),
+                            "rel_l2_vs_fp64": float(math.sqrt(err) / ydn)}
+        log(f"[bench] split n4096 {names[mode]}: {ms * 1e3:.1f} us rel-L2 {out[names[mode]]['rel_l2_vs_fp64']:.3e}")
+        p.destroy()
+    del x, y, yd
+    torch.cuda.empty_cache()
+    return out
+
+
 def copy_ceiling(torch, mib=256, reps=20):
     """Plain device-to-device copy of a known size: the practical HBM
     ceiling on this box, and the calibration kernel for FETCH/WRITE_SIZE."""
@@ -365,6 +439,8 @@ def main():
     log("[bench] start uneven batch (20,000 items) fp32")
     ru = bench_uneven(torch, fftz, args.steps, args.warmup)
     log(f"[bench] uneven: {ru['ms_per_execute']:.3f} ms {ru['gflops']:.1f} GFLOP/s frac {ru['frac_of_hbm']:.3f}")
+    log("[bench] start split-precision matrix-core vs plain, N=4096 batch 16384")
+    rs = bench_split(torch, fftz, args.steps, args.warmup)
     copy_bps = copy_ceiling(torch)
     log(f"[bench] copy ceiling {copy_bps / 1e9:.0f} GB/s")
     base, extra = (None, {})
@@ -399,6 +475,7 @@ def main():
         "batched_n1024_fp32": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r10.items()
                                if not k.startswith("_")},
         "uneven_batch_fp32": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in ru.items()},
+        "split_mfma_n4096_b16384": rs,
     }
     line["copy_ceiling_GBps"] = round(copy_bps / 1e9, 1)
     line.update(extra)

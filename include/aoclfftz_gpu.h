@@ -37,6 +37,19 @@ aoclfftz_error_type aoclfftz_gpu_synchronize(FFTZ_VOID *handle);
  * replay it (enable = 1), or launch kernels directly (enable = 0). */
 aoclfftz_error_type aoclfftz_gpu_set_graph(FFTZ_VOID *handle, FFTZ_INT32 enable);
 
+/* Operand precision of the radix-16 stages of complex64 passes of length 256
+ * or 4096 (the all-16 radix plan; the batched N = 4096 and N = 256 cases):
+ *   0  plain FP32 Stockham codelets on the vector ALUs (default);
+ *   1  each radix-16 stage as a DFT-matrix GEMM on the matrix cores with
+ *      FP32 operands;
+ *   2  the same GEMM with operands split into bf16 hi + lo, three products
+ *      (hi*hi + hi*lo + lo*hi), FP32 accumulation;
+ *   3  operands scaled per column by a power of two and split into fp16
+ *      hi + lo, three products, FP32 accumulation.
+ * Re-plans the handle.  AOCLFFTZ_INVALID_INPUT (plan unchanged) when no pass
+ * of the plan qualifies.  Not to be called while the handle executes. */
+aoclfftz_error_type aoclfftz_gpu_set_compute_mode(FFTZ_VOID *handle, FFTZ_INT32 mode);
+
 /* Human-readable plan summary (pass list with lengths, radices, tiles).  The
  * string lives as long as the handle. */
 const FFTZ_CHAR *aoclfftz_gpu_plan_info(FFTZ_VOID *handle);
