@@ -33,15 +33,30 @@ struct LineIdx {
     uint32_t l0;
 };
 
-__device__ __forceinline__ LineIdx line_index(const PassDesc& p, uint32_t l)
+// The buffers and line set a pass body works on: the descriptor's own for a
+// single-size pass, the queue entry's for the uneven-batch kernel (which reads
+// its size class's descriptor in place and never patches it).
+struct TileIo {
+    const void* in;
+    void* out;
+    int64_t nlines;
+    int64_t ils0, ols0;  // line strides of level 0
+};
+
+__device__ __forceinline__ TileIo io_of(const PassDesc& p)
+{
+    return TileIo{p.in, p.out, p.nlines, p.ils[0], p.ols[0]};
+}
+
+__device__ __forceinline__ LineIdx line_index(const PassDesc& p, const TileIo& io, uint32_t l)
 {
     uint32_t r = fdiv(l, p.ln0);
     uint32_t l0 = l - r * p.ln0.d;
     uint32_t l2 = fdiv(r, p.ln1);
     uint32_t l1 = r - l2 * p.ln1.d;
     LineIdx li;
-    li.ioff = (int64_t)l0 * p.ils[0] + (int64_t)l1 * p.ils[1] + (int64_t)l2 * p.ils[2];
-    li.ooff = (int64_t)l0 * p.ols[0] + (int64_t)l1 * p.ols[1] + (int64_t)l2 * p.ols[2];
+    li.ioff = (int64_t)l0 * io.ils0 + (int64_t)l1 * p.ils[1] + (int64_t)l2 * p.ils[2];
+    li.ooff = (int64_t)l0 * io.ols0 + (int64_t)l1 * p.ols[1] + (int64_t)l2 * p.ols[2];
     li.l0 = l0;
     return li;
 }
@@ -66,18 +81,18 @@ __device__ __forceinline__ cplx<double> tw4_root(const PassDesc& p, int64_t q)
 // t = tid / C): a wavefront walks the same element of C adjacent lines.
 // ---------------------------------------------------------------------------
 template <typename R, int E>
-__device__ __forceinline__ void load_tile(const PassDesc& p, cplx<R>* lds, int64_t l_base, int tid)
+__device__ __forceinline__ void load_tile(const PassDesc& p, const TileIo& io, cplx<R>* lds, int64_t l_base, int tid)
 {
     const bool col = (p.flags & kInCol) != 0;
     const int c = col ? (tid & (p.C - 1)) : (tid >> p.tl_shift);
     const int t = col ? (tid >> p.c_shift) : (tid & (p.TL - 1));
     const int64_t l = l_base + c;
-    if (l >= p.nlines) return;
-    LineIdx li = line_index(p, (uint32_t)l);
+    if (l >= io.nlines) return;
+    LineIdx li = line_index(p, io, (uint32_t)l);
     cplx<R>* line = lds + c * p.LP;
     if (p.flags & kRealIn) {
         // R2C: element j of the L-point complex line is (x[2j], x[2j+1])
-        const R* src = reinterpret_cast<const R*>(p.in) + li.ioff;
+        const R* src = reinterpret_cast<const R*>(io.in) + li.ioff;
         cplx<R> v[E];
 #pragma unroll
         for (int k = 0; k < E; k++) {
@@ -94,7 +109,7 @@ __device__ __forceinline__ void load_tile(const PassDesc& p, cplx<R>* lds, int64
         }
         return;
     }
-    const cplx<R>* src = reinterpret_cast<const cplx<R>*>(p.in) + li.ioff;
+    const cplx<R>* src = reinterpret_cast<const cplx<R>*>(io.in) + li.ioff;
     const int nv = p.nvalid_in;
     const int ncover = (p.flags & kC2RPre) ? p.L + 1 : p.L;
     cplx<R> v[E + 1];
@@ -224,15 +239,15 @@ __device__ __forceinline__ void run_stages(const PassDesc& p, cplx<R>* lds, int 
 // bins k and L-k of the N = 2L real transform from Z_k, Z_{L-k}
 // (core/kernels/non_dft/real_packed/packed_rdft_common.h:19-60).
 template <typename R>
-__device__ __forceinline__ void r2c_recombine_store(const PassDesc& p, const cplx<R>* lds, int64_t l_base, int tid)
+__device__ __forceinline__ void r2c_recombine_store(const PassDesc& p, const TileIo& io, const cplx<R>* lds, int64_t l_base, int tid)
 {
     const int c = tid >> p.tl_shift;
     const int t = tid & (p.TL - 1);
     const int64_t l = l_base + c;
-    if (l >= p.nlines) return;
-    LineIdx li = line_index(p, (uint32_t)l);
+    if (l >= io.nlines) return;
+    LineIdx li = line_index(p, io, (uint32_t)l);
     const cplx<R>* line = lds + c * p.LP;
-    cplx<R>* dst = reinterpret_cast<cplx<R>*>(p.out) + li.ooff;
+    cplx<R>* dst = reinterpret_cast<cplx<R>*>(io.out) + li.ooff;
     const cplx<R>* rtw = reinterpret_cast<const cplx<R>*>(p.rtw);
     const int L = p.L;
     const R half = R(0.5);
@@ -262,14 +277,14 @@ __device__ __forceinline__ void r2c_recombine_store(const PassDesc& p, const cpl
 }
 
 template <typename R, int E>
-__device__ __forceinline__ void store_tile(const PassDesc& p, const cplx<R>* lds, int64_t l_base, int tid)
+__device__ __forceinline__ void store_tile(const PassDesc& p, const TileIo& io, const cplx<R>* lds, int64_t l_base, int tid)
 {
     const bool col = (p.flags & kOutCol) != 0;
     const int c = col ? (tid & (p.C - 1)) : (tid >> p.tl_shift);
     const int t = col ? (tid >> p.c_shift) : (tid & (p.TL - 1));
     const int64_t l = l_base + c;
-    if (l >= p.nlines) return;
-    LineIdx li = line_index(p, (uint32_t)l);
+    if (l >= io.nlines) return;
+    LineIdx li = line_index(p, io, (uint32_t)l);
     const cplx<R>* line = lds + c * p.LP;
     cplx<R> v[E];
 #pragma unroll
@@ -305,7 +320,7 @@ __device__ __forceinline__ void store_tile(const PassDesc& p, const cplx<R>* lds
     if (p.flags & kRealOut) {
         // C2R: element j is (x[2j], x[2j+1]); stored conjugated-domain value
         // was already flipped back by kConjOut.
-        R* dst = reinterpret_cast<R*>(p.out) + li.ooff;
+        R* dst = reinterpret_cast<R*>(io.out) + li.ooff;
 #pragma unroll
         for (int k = 0; k < E; k++) {
             const int e = t + k * p.TL;
@@ -316,7 +331,7 @@ __device__ __forceinline__ void store_tile(const PassDesc& p, const cplx<R>* lds
         }
         return;
     }
-    cplx<R>* dst = reinterpret_cast<cplx<R>*>(p.out) + li.ooff;
+    cplx<R>* dst = reinterpret_cast<cplx<R>*>(io.out) + li.ooff;
 #pragma unroll
     for (int k = 0; k < E; k++) {
         const int e = t + k * p.TL;
@@ -524,22 +539,22 @@ __device__ __forceinline__ void single_stage_direct(const PassDesc& p, int t, co
 
 // Complex-in pass with register-direct first / last stages.
 template <typename R, bool ALLRAD, int E>
-__device__ __forceinline__ void pass_body_direct(const PassDesc& p, cplx<R>* lds, int64_t l_base, int tid)
+__device__ __forceinline__ void pass_body_direct(const PassDesc& p, const TileIo& io, cplx<R>* lds, int64_t l_base, int tid)
 {
     const bool in_col = (p.flags & kInCol) != 0, out_col = (p.flags & kOutCol) != 0;
     const int ci = in_col ? (tid & (p.C - 1)) : (tid >> p.tl_shift);
     const int ti = in_col ? (tid >> p.c_shift) : (tid & (p.TL - 1));
     const int co = out_col ? (tid & (p.C - 1)) : (tid >> p.tl_shift);
     const int to = out_col ? (tid >> p.c_shift) : (tid & (p.TL - 1));
-    const bool live_in = ci < p.C && (l_base + ci) < p.nlines;
-    const bool live_out = co < p.C && (l_base + co) < p.nlines;
+    const bool live_in = ci < p.C && (l_base + ci) < io.nlines;
+    const bool live_out = co < p.C && (l_base + co) < io.nlines;
     const cplx<R>* src = nullptr;
-    if (live_in) src = reinterpret_cast<const cplx<R>*>(p.in) + line_index(p, (uint32_t)(l_base + ci)).ioff;
+    if (live_in) src = reinterpret_cast<const cplx<R>*>(io.in) + line_index(p, io, (uint32_t)(l_base + ci)).ioff;
     cplx<R>* dst = nullptr;
     uint32_t l0 = 0;
     if (live_out) {
-        LineIdx li = line_index(p, (uint32_t)(l_base + co));
-        dst = reinterpret_cast<cplx<R>*>(p.out) + li.ooff;
+        LineIdx li = line_index(p, io, (uint32_t)(l_base + co));
+        dst = reinterpret_cast<cplx<R>*>(io.out) + li.ooff;
         l0 = li.l0;
     }
     const int S = p.nst;
@@ -548,8 +563,8 @@ __device__ __forceinline__ void pass_body_direct(const PassDesc& p, cplx<R>* lds
         cplx<R>* d1 = nullptr;
         uint32_t l01 = 0;
         if (live_in) {
-            LineIdx li = line_index(p, (uint32_t)(l_base + ci));
-            d1 = reinterpret_cast<cplx<R>*>(p.out) + li.ooff;
+            LineIdx li = line_index(p, io, (uint32_t)(l_base + ci));
+            d1 = reinterpret_cast<cplx<R>*>(io.out) + li.ooff;
             l01 = li.l0;
         }
         FFTZ_RADIX_SWITCH(p.radix[0], ALLRAD, (single_stage_direct<R, E, RAD>(p, ti, src, d1, l01, live_in)))
@@ -563,14 +578,14 @@ __device__ __forceinline__ void pass_body_direct(const PassDesc& p, cplx<R>* lds
 }
 
 template <typename R, bool ALLRAD, int E>
-__device__ __forceinline__ void pass_body(const PassDesc& p, cplx<R>* lds, int64_t l_base, int tid)
+__device__ __forceinline__ void pass_body(const PassDesc& p, const TileIo& io, cplx<R>* lds, int64_t l_base, int tid)
 {
     // (a zero-stage plan, L = 1, is a strided copy: the staged path below)
     if (!(p.flags & (kRealIn | kC2RPre | kR2CPost)) && p.nst >= 1) {
-        pass_body_direct<R, ALLRAD, E>(p, lds, l_base, tid);
+        pass_body_direct<R, ALLRAD, E>(p, io, lds, l_base, tid);
         return;
     }
-    load_tile<R, E>(p, lds, l_base, tid);
+    load_tile<R, E>(p, io, lds, l_base, tid);
     __syncthreads();
     if (p.flags & kC2RPre) {
         c2r_separate<R>(p, lds, tid);
@@ -578,9 +593,9 @@ __device__ __forceinline__ void pass_body(const PassDesc& p, cplx<R>* lds, int64
     }
     run_stages<R, ALLRAD, E>(p, lds, tid, 0, p.nst);
     if (p.flags & kR2CPost) {
-        r2c_recombine_store<R>(p, lds, l_base, tid);
+        r2c_recombine_store<R>(p, io, lds, l_base, tid);
     } else {
-        store_tile<R, E>(p, lds, l_base, tid);
+        store_tile<R, E>(p, io, lds, l_base, tid);
     }
 }
 
@@ -593,7 +608,7 @@ __global__ void __launch_bounds__(MAXT) k_fft_pass(const PassDesc p_arg)
     cplx<R>* lds = reinterpret_cast<cplx<R>*>(smem_raw);
     const int tid = threadIdx.x;
     const int64_t l_base = (int64_t)tile_of_block(p) * p.C;
-    pass_body<R, ALLRAD, E>(p, lds, l_base, tid);
+    pass_body<R, ALLRAD, E>(p, io_of(p), lds, l_base, tid);
 }
 
 // ---------------------------------------------------------------------------
@@ -609,21 +624,21 @@ __global__ void __launch_bounds__(MAXT) k_fft_pass(const PassDesc p_arg)
 // sweeps (pre_mul, memset, FFT, mul, IFFT, post_mul) in one HBM round trip.
 // ---------------------------------------------------------------------------
 template <typename R, bool ALLRAD, int E>
-__device__ __forceinline__ void bluestein_body(const PassDesc& p, cplx<R>* lds, int64_t l_base, int tid)
+__device__ __forceinline__ void bluestein_body(const PassDesc& p, const TileIo& io, cplx<R>* lds, int64_t l_base, int tid)
 {
     const int c = tid >> p.tl_shift;
     const int t = tid & (p.TL - 1);
     const int64_t l = l_base + c;
-    const bool live = l < p.nlines;
+    const bool live = l < io.nlines;
     LineIdx li = {0, 0, 0};
-    if (live) li = line_index(p, (uint32_t)l);
+    if (live) li = line_index(p, io, (uint32_t)l);
     cplx<R>* line = lds + c * p.LP;
     const cplx<R>* chirp = reinterpret_cast<const cplx<R>*>(p.chirp);
     const cplx<R>* spec = reinterpret_cast<const cplx<R>*>(p.spec);
     const int n = p.nvalid_in;
     const int M = p.L;
     if (live) {
-        const cplx<R>* src = reinterpret_cast<const cplx<R>*>(p.in) + li.ioff;
+        const cplx<R>* src = reinterpret_cast<const cplx<R>*>(io.in) + li.ioff;
         for (int e = t; e < M; e += p.TL) {
             cplx<R> a = {R(0), R(0)};
             if (e < n) {
@@ -645,7 +660,7 @@ __device__ __forceinline__ void bluestein_body(const PassDesc& p, cplx<R>* lds, 
     __syncthreads();
     run_stages<R, ALLRAD, E>(p, lds, tid, 0, p.nst);
     if (live) {
-        cplx<R>* dst = reinterpret_cast<cplx<R>*>(p.out) + li.ooff;
+        cplx<R>* dst = reinterpret_cast<cplx<R>*>(io.out) + li.ooff;
         for (int k = t; k < n; k += p.TL) {
             cplx<R> w = line[lds_pad(k)];
             cplx<R> b = cconj(chirp[k]);
@@ -781,17 +796,17 @@ __device__ __forceinline__ void bs_final(const PassDesc& p, const cplx<R>* lds, 
 }
 
 template <typename R, bool ALLRAD, int E>
-__device__ __forceinline__ void bluestein_body_direct(const PassDesc& p, cplx<R>* lds, int64_t l_base, int tid)
+__device__ __forceinline__ void bluestein_body_direct(const PassDesc& p, const TileIo& io, cplx<R>* lds, int64_t l_base, int tid)
 {
     const int c = tid >> p.tl_shift;
     const int t = tid & (p.TL - 1);
-    const bool live = c < p.C && (l_base + c) < p.nlines;
+    const bool live = c < p.C && (l_base + c) < io.nlines;
     const cplx<R>* src = nullptr;
     cplx<R>* dst = nullptr;
     if (live) {
-        LineIdx li = line_index(p, (uint32_t)(l_base + c));
-        src = reinterpret_cast<const cplx<R>*>(p.in) + li.ioff;
-        dst = reinterpret_cast<cplx<R>*>(p.out) + li.ooff;
+        LineIdx li = line_index(p, io, (uint32_t)(l_base + c));
+        src = reinterpret_cast<const cplx<R>*>(io.in) + li.ioff;
+        dst = reinterpret_cast<cplx<R>*>(io.out) + li.ooff;
     }
     const int S = p.nst;
     FFTZ_RADIX_SWITCH(p.radix[0], ALLRAD, (bs_first<R, E, RAD>(p, lds, c, t, src, live)))
@@ -810,10 +825,11 @@ __global__ void __launch_bounds__(MAXT) k_bluestein(const PassDesc p_arg)
     const PassDesc& p = kernarg_desc();
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     cplx<R>* lds = reinterpret_cast<cplx<R>*>(smem_raw);
+    const TileIo io = io_of(p);
     if (p.nst >= 2)
-        bluestein_body_direct<R, ALLRAD, E>(p, lds, (int64_t)blockIdx.x * p.C, threadIdx.x);
+        bluestein_body_direct<R, ALLRAD, E>(p, io, lds, (int64_t)blockIdx.x * p.C, threadIdx.x);
     else
-        bluestein_body<R, ALLRAD, E>(p, lds, (int64_t)blockIdx.x * p.C, threadIdx.x);
+        bluestein_body<R, ALLRAD, E>(p, io, lds, (int64_t)blockIdx.x * p.C, threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -826,48 +842,46 @@ __global__ void __launch_bounds__(MAXT) k_bluestein(const PassDesc p_arg)
 // workgroup that runs it, so results are bitwise identical run to run.
 // ---------------------------------------------------------------------------
 template <typename R, int E, int MAXT>
-__global__ void __launch_bounds__(MAXT) k_batch(const BatchArgs a)
+__global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : 2)))
+k_batch(const BatchArgs a)
 {
-    // one dynamic LDS region, laid out explicitly: [descriptor | tile index |
-    // tile data], each part 16-byte aligned (static __shared__ variables next
-    // to a dynamic array would leave the array misaligned for 8/16-byte LDS
-    // accesses)
+    // dynamic LDS: [tile index | tile data].  The tile index is made uniform
+    // with readfirstlane, and the queue entry and its size class's descriptor
+    // are read in place from the plan tables through the constant address
+    // space (read-only for the whole launch), i.e. with scalar loads.
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
-    PassDesc& pd = *reinterpret_cast<PassDesc*>(smem_raw);
-    int& s_tile = *reinterpret_cast<int*>(smem_raw + kBatchHdrBytes - 16);
+    int& s_tile = *reinterpret_cast<int*>(smem_raw);
     cplx<R>* lds = reinterpret_cast<cplx<R>*>(smem_raw + kBatchHdrBytes);
     const int tid = threadIdx.x;
-    constexpr int kWords = sizeof(PassDesc) / 4;
+    const int esz = 2 * (int)sizeof(R);
+    if (tid == 0) s_tile = (int)atomicAdd(a.counter, 1u);
+    __syncthreads();
     for (;;) {
-        if (tid == 0) s_tile = (int)atomicAdd(a.counter, 1u);
-        __syncthreads();
-        const int t = s_tile;
+        const int t = __builtin_amdgcn_readfirstlane(s_tile);
         if (t >= a.ntiles) break;
-        const BatchTile bt = a.tiles[t];
-        // word-parallel copy through memcpy (char-typed accesses may alias
-        // the descriptor's pointer / int64 / double fields; plain uint32_t
-        // stores would let the compiler keep the previous tile's values)
-        const char* src = reinterpret_cast<const char*>(a.descs + bt.desc);
-        char* dstb = reinterpret_cast<char*>(&pd);
-        for (int w = tid; w < kWords; w += MAXT) __builtin_memcpy(dstb + 4 * w, src + 4 * w, 4);
-        __syncthreads();
-        if (tid == 0) {
-            const int esz = 2 * (int)sizeof(R);
-            pd.in = reinterpret_cast<const char*>(a.in) + bt.in_off * esz;
-            pd.out = reinterpret_cast<char*>(a.out) + bt.out_off * esz;
-            pd.nlines = bt.nlines;
-            pd.ils[0] = bt.dist;
-            pd.ols[0] = bt.dist;
+        // the next tile's index is fetched now and published at the end, so
+        // the atomic's round trip overlaps this tile's work
+        int nxt = 0;
+        if (tid == 0) nxt = (int)atomicAdd(a.counter, 1u);
+        const BatchTile& bt = *(const BatchTile*)((const __attribute__((address_space(4))) BatchTile*)a.tiles + t);
+        const PassDesc& p = *(const PassDesc*)((const __attribute__((address_space(4))) PassDesc*)a.descs + bt.desc);
+        const TileIo io{reinterpret_cast<const char*>(a.in) + bt.in_off * esz,
+                        reinterpret_cast<char*>(a.out) + bt.out_off * esz, (int64_t)bt.nlines, bt.dist, bt.dist};
+        const bool blue = bt.bluestein != 0;
+        const int C = p.C;
+        // a queue entry covers one or more rounds of C lines
+        for (int64_t lb = 0; lb < io.nlines; lb += C) {
+            if (blue) {
+                if (p.nst >= 2)
+                    bluestein_body_direct<R, true, E>(p, io, lds, lb, tid);
+                else
+                    bluestein_body<R, true, E>(p, io, lds, lb, tid);
+            } else {
+                pass_body<R, true, E>(p, io, lds, lb, tid);
+            }
+            __syncthreads();
         }
-        __syncthreads();
-        if (bt.bluestein) {
-            if (pd.nst >= 2)
-                bluestein_body_direct<R, true, E>(pd, lds, 0, tid);
-            else
-                bluestein_body<R, true, E>(pd, lds, 0, tid);
-        } else {
-            pass_body<R, true, E>(pd, lds, 0, tid);
-        }
+        if (tid == 0) s_tile = nxt;
         __syncthreads();
     }
 }
@@ -897,9 +911,11 @@ const void* kernel_symbol_batch(int prec_bytes, int E, int maxt)
 {
     if (prec_bytes == 4) {
         if (E == 8) return maxt <= 256 ? (const void*)&k_batch<float, 8, 256> : (const void*)&k_batch<float, 8, 1024>;
+        if (maxt == 512) return (const void*)&k_batch<float, 16, 512>;
         return maxt <= 256 ? (const void*)&k_batch<float, 16, 256> : (const void*)&k_batch<float, 16, 1024>;
     }
     if (E == 8) return maxt <= 256 ? (const void*)&k_batch<double, 8, 256> : (const void*)&k_batch<double, 8, 1024>;
+    if (maxt == 512) return (const void*)&k_batch<double, 16, 512>;
     return maxt <= 256 ? (const void*)&k_batch<double, 16, 256> : (const void*)&k_batch<double, 16, 1024>;
 }
 

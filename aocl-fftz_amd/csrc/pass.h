@@ -108,9 +108,9 @@ struct BatchTile {
     int64_t dist;
 };
 
-// Bytes at the start of the batch kernel's dynamic LDS that hold the tile's
-// pass descriptor and the tile index (rounded to 256).
-constexpr int kBatchHdrBytes = (int)(((sizeof(PassDesc) + 16 + 255) / 256) * 256);
+// Bytes at the start of the batch kernel's dynamic LDS that hold the tile
+// index (rounded to 256 so the tile data stays aligned).
+constexpr int kBatchHdrBytes = 256;
 
 struct BatchArgs {
     const PassDesc* descs;

@@ -1005,6 +1005,8 @@ Plan* batch_plan_create(const std::vector<BatchItem>& items, int prec, int dir, 
     const size_t static_lds = kBatchHdrBytes;
     const size_t small_budget = (size_t)env_int("FFTZ_BATCH_SMALL_LDS", 64 * 1024);
     const size_t big_budget = dev.lds_per_block - static_lds;
+    // medium tiles: 512 threads, at most half the LDS, two workgroups per CU
+    const size_t medium_budget = dev.lds_per_block / 2 - static_lds;
     std::map<int64_t, int> size_index;
     std::vector<SizeInfo> sizes;
     int64_t in_ext = 0, out_ext = 0;
@@ -1025,12 +1027,16 @@ Plan* batch_plan_create(const std::vector<BatchItem>& items, int prec, int dir, 
         SizeInfo si;
         const int64_t L = smooth13(it.n) ? it.n : bluestein_length(it.n);
         const bool small = next_pow2((L + 7) / 8) <= 256 && (size_t)(lds_pad((int)L) + 2) * esize <= small_budget / 2;
-        if (!batch_size_desc(tb, prec, it.n, small ? 8 : 16, small ? 256 : 1024, small ? small_budget : big_budget,
-                             si, err)) {
+        const bool medium = !small && next_pow2((L + 15) / 16) <= 512 &&
+                            (size_t)(lds_pad((int)L) + 2) * esize <= medium_budget;
+        const int cls = small ? 0 : (medium ? 1 : 2);
+        const int cE[3] = {8, 16, 16}, cT[3] = {256, 512, 1024};
+        const size_t cB[3] = {small_budget, medium_budget, big_budget};
+        if (!batch_size_desc(tb, prec, it.n, cE[cls], cT[cls], cB[cls], si, err)) {
             delete plan;
             return nullptr;
         }
-        si.cls = small ? 0 : 1;
+        si.cls = cls;
         size_index[it.n] = (int)sizes.size();
         sizes.push_back(si);
     }
@@ -1038,15 +1044,22 @@ Plan* batch_plan_create(const std::vector<BatchItem>& items, int prec, int dir, 
     plan->out_extent_bytes = out_ext * esize;
     plan->flops = flops;
     // tiles per class, cost-sorted (largest first)
-    std::vector<std::vector<std::pair<double, BatchTile>>> tiles(2);
+    // a queue entry covers up to `rounds` rounds of C lines (about 128 KiB of
+    // input) so the per-entry cost (queue atomic, descriptor copy, barriers)
+    // is amortised
+    const int64_t entry_bytes = env_int("FFTZ_BATCH_ENTRY_BYTES", 128 * 1024);
+    std::vector<std::vector<std::pair<double, BatchTile>>> tiles(3);
     for (const BatchItem& it : items) {
         if (it.batch == 0) continue;
         const SizeInfo& si = sizes[(size_t)size_index[it.n]];
         const int64_t dist = it.dist ? it.dist : it.n;
-        for (int64_t l0 = 0; l0 < it.batch; l0 += si.C) {
+        const int64_t per_round = (int64_t)si.C * si.n * esize;
+        const int64_t rounds = std::max<int64_t>(1, std::min<int64_t>(16, entry_bytes / std::max<int64_t>(1, per_round)));
+        const int64_t step = (int64_t)si.C * rounds;
+        for (int64_t l0 = 0; l0 < it.batch; l0 += step) {
             BatchTile bt{};
             bt.desc = (uint32_t)size_index[it.n];
-            bt.nlines = (uint32_t)std::min<int64_t>(si.C, it.batch - l0);
+            bt.nlines = (uint32_t)std::min<int64_t>(step, it.batch - l0);
             bt.bluestein = si.bluestein ? 1u : 0u;
             bt.in_off = it.in_off + l0 * dist;
             bt.out_off = it.out_off + l0 * dist;
@@ -1063,18 +1076,19 @@ Plan* batch_plan_create(const std::vector<BatchItem>& items, int prec, int dir, 
     std::vector<PassDesc> descs;
     for (const SizeInfo& si : sizes) descs.push_back(si.d);
     plan->descs_off = tb.add(descs.data(), std::max<size_t>(1, descs.size()) * sizeof(PassDesc));
-    for (int c = 0; c < 2; c++) {
+    for (int c = 0; c < 3; c++) {
         auto& tl = tiles[(size_t)c];
         std::stable_sort(tl.begin(), tl.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
         BatchClass bc;
         bc.E = c == 0 ? 8 : 16;
-        bc.maxt = c == 0 ? 256 : 1024;
+        bc.maxt = c == 0 ? 256 : (c == 1 ? 512 : 1024);
         bc.ntiles = (int)tl.size();
         size_t lds = 0;
         for (const SizeInfo& si : sizes)
             if (si.cls == c) lds = std::max(lds, (size_t)si.C * si.d.LP * esize);
         bc.lds_bytes = (uint32_t)(lds + kBatchHdrBytes);
-        const int per_cu = c == 0 ? env_int("FFTZ_BATCH_WG_SMALL", 8) : env_int("FFTZ_BATCH_WG_BIG", 1);
+        const int per_cu = c == 0 ? env_int("FFTZ_BATCH_WG_SMALL", 8)
+                           : c == 1 ? env_int("FFTZ_BATCH_WG_MEDIUM", 2) : env_int("FFTZ_BATCH_WG_BIG", 1);
         bc.grid = (uint32_t)std::max<int64_t>(1, std::min<int64_t>(bc.ntiles, (int64_t)dev.cus * per_cu));
         if (bc.ntiles) {
             std::vector<BatchTile> v;
@@ -1085,10 +1099,12 @@ Plan* batch_plan_create(const std::vector<BatchItem>& items, int prec, int dir, 
         plan->bclasses.push_back(bc);
     }
     std::ostringstream os;
-    os << "uneven batch: " << plan->n_transforms << " transforms, " << sizes.size() << " lengths; tiles small="
-       << plan->bclasses[0].ntiles << " (grid " << plan->bclasses[0].grid << ", lds " << plan->bclasses[0].lds_bytes
-       << ") large=" << plan->bclasses[1].ntiles << " (grid " << plan->bclasses[1].grid << ", lds "
-       << plan->bclasses[1].lds_bytes << "); ";
+    static const char* kClassName[3] = {"small", "medium", "large"};
+    os << "uneven batch: " << plan->n_transforms << " transforms, " << sizes.size() << " lengths; tiles";
+    for (int c = 0; c < 3; c++)
+        os << " " << kClassName[c] << "=" << plan->bclasses[(size_t)c].ntiles << " (grid " << plan->bclasses[(size_t)c].grid
+           << ", lds " << plan->bclasses[(size_t)c].lds_bytes << ")";
+    os << "; ";
     plan->description = os.str();
     if (dry_run) return plan;
     if (hipMalloc(&plan->tables, plan->host_tables.size()) != hipSuccess) {
@@ -1113,8 +1129,8 @@ Plan* batch_plan_create(const std::vector<BatchItem>& items, int prec, int dir, 
         delete plan;
         return nullptr;
     }
-    for (int c = 0; c < 2; c++)
-        (void)hipFuncSetAttribute(kernel_symbol_batch(prec, plan->bclasses[(size_t)c].E, plan->bclasses[(size_t)c].maxt),
+    for (size_t c = 0; c < plan->bclasses.size(); c++)
+        (void)hipFuncSetAttribute(kernel_symbol_batch(prec, plan->bclasses[c].E, plan->bclasses[c].maxt),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)dev.lds_per_block);
     plan->host_tables.clear();
     plan->host_tables.shrink_to_fit();

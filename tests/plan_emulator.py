@@ -163,8 +163,9 @@ def run_batch(plan, x, direction=0):
         for desc, nl, blu, ioff, ooff, dist in cls["tiles"]:
             s = sizes[desc]
             n = s["nvalid"]
-            assert nl <= s["C"] and s["C"] * s["TL"] <= cls["maxt"]
-            assert s["C"] * s["LP"] * esz + 768 <= cls["lds"] <= 163840
+            # an entry covers 1..16 rounds of C lines
+            assert nl <= 16 * s["C"] and s["C"] * s["TL"] <= cls["maxt"]
+            assert s["C"] * s["LP"] * esz + 256 <= cls["lds"] <= 163840  # 256 B header: tile index
             assert bool(blu) == (s["L"] != n)
             for l in range(nl):
                 i0 = ioff + l * dist
