@@ -507,6 +507,44 @@ const FFTZ_CHAR* aoclfftz_gpu_plan_info(FFTZ_VOID* handle)
     return h->info.c_str();
 }
 
+}  // extern "C"
+
+namespace fftz {
+
+bool spec_from_desc(const void* problem, int prec_bytes, bool ilp64, ProblemSpec& spec, const void** in, void** out,
+                    const char* func)
+{
+    if (!problem || (prec_bytes != 4 && prec_bytes != 8)) return false;
+    if (ilp64) {
+        auto* pd = static_cast<aoclfftz_prob_desc_d_64_*>(const_cast<void*>(problem));
+        int rc = validate<aoclfftz_prob_desc_d_64_, aoclfftz_dim_t_64_>(pd);
+        if (rc != AOCLFFTZ_SUCCESS) {
+            report_error(func, "Problem descriptor validation failed with %s", status_name(rc));
+            return false;
+        }
+        spec = spec_of(pd, prec_bytes);
+        *in = pd->in;
+        *out = pd->out;
+    } else {
+        auto* pd = static_cast<aoclfftz_prob_desc_d*>(const_cast<void*>(problem));
+        int rc = validate<aoclfftz_prob_desc_d, aoclfftz_dim_t>(pd);
+        if (rc != AOCLFFTZ_SUCCESS) {
+            report_error(func, "Problem descriptor validation failed with %s", status_name(rc));
+            return false;
+        }
+        spec = spec_of(pd, prec_bytes);
+        *in = pd->in;
+        *out = pd->out;
+    }
+    return true;
+}
+
+void report_msg(const char* func, const char* msg) { report_error(func, "%s", msg); }
+
+}  // namespace fftz
+
+extern "C" {
+
 FFTZ_INT64 aoclfftz_gpu_plan_json(FFTZ_VOID* problem, FFTZ_INT32 precision_bytes, FFTZ_INT32 ilp64,
                                   FFTZ_CHAR* buf, FFTZ_INT64 buflen)
 {

@@ -67,6 +67,9 @@ EXPORTS = (
     "aoclfftz_execute", "aoclfftz_execute_io", "aoclfftz_destroy", "aoclfftz_version",
     "aoclfftz_gpu_set_stream", "aoclfftz_gpu_synchronize", "aoclfftz_gpu_set_graph",
     "aoclfftz_gpu_set_compute_mode",
+    "aoclfftz_gpu_server_create", "aoclfftz_gpu_server_submit", "aoclfftz_gpu_server_test",
+    "aoclfftz_gpu_server_wait", "aoclfftz_gpu_server_stats", "aoclfftz_gpu_server_reset_stats",
+    "aoclfftz_gpu_server_destroy",
     "aoclfftz_gpu_plan_info", "aoclfftz_gpu_plan_flops", "aoclfftz_gpu_plan_device_bytes",
     "aoclfftz_gpu_plan_json", "aoclfftz_gpu_profile", "aoclfftz_gpu_profile_read",
     "aoclfftz_gpu_plan_launches", "aoclfftz_gpu_execute_many", "aoclfftz_gpu_batch_setup",
@@ -110,6 +113,20 @@ def load():
     lib.aoclfftz_gpu_set_graph.restype = ctypes.c_int
     lib.aoclfftz_gpu_set_compute_mode.argtypes = [vp, ctypes.c_int32]
     lib.aoclfftz_gpu_set_compute_mode.restype = ctypes.c_int
+    lib.aoclfftz_gpu_server_create.argtypes = [ctypes.c_int64, ctypes.c_int32]
+    lib.aoclfftz_gpu_server_create.restype = vp
+    lib.aoclfftz_gpu_server_submit.argtypes = [vp, vp, ctypes.c_int32]
+    lib.aoclfftz_gpu_server_submit.restype = ctypes.c_int64
+    lib.aoclfftz_gpu_server_test.argtypes = [vp, ctypes.c_int64]
+    lib.aoclfftz_gpu_server_test.restype = ctypes.c_int32
+    lib.aoclfftz_gpu_server_wait.argtypes = [vp, ctypes.c_int64, ctypes.POINTER(ctypes.c_double)]
+    lib.aoclfftz_gpu_server_wait.restype = ctypes.c_int32
+    lib.aoclfftz_gpu_server_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.c_int32]
+    lib.aoclfftz_gpu_server_stats.restype = ctypes.c_int32
+    lib.aoclfftz_gpu_server_reset_stats.argtypes = [vp]
+    lib.aoclfftz_gpu_server_reset_stats.restype = None
+    lib.aoclfftz_gpu_server_destroy.argtypes = [vp]
+    lib.aoclfftz_gpu_server_destroy.restype = None
     lib.aoclfftz_gpu_plan_info.argtypes = [vp]
     lib.aoclfftz_gpu_plan_info.restype = ctypes.c_char_p
     lib.aoclfftz_gpu_plan_flops.argtypes = [vp]
@@ -278,6 +295,52 @@ class BatchPlan(Plan):
                                                       direction)
         if not self.handle:
             raise RuntimeError("aoclfftz_gpu_batch_setup failed")
+
+
+class Server:
+    """Serving runtime (aoclfftz_gpu_server_*): submit requests from Python,
+    wait for them, read latency / throughput statistics."""
+
+    STATS = ("completed", "failed", "p50_ms", "p99_ms", "max_ms", "transforms_per_s", "gflops",
+             "peak_footprint_bytes", "plans_resident", "queue_peak", "span_s")
+
+    def __init__(self, hbm_budget_bytes: int, max_inflight: int = 8):
+        self.handle = load().aoclfftz_gpu_server_create(int(hbm_budget_bytes), int(max_inflight))
+        if not self.handle:
+            raise RuntimeError("aoclfftz_gpu_server_create failed")
+
+    def submit(self, dims, vecs, precision: str = "f", direction: int = 0, fft_type: int = 0,
+               inp=None, out=None, inplace: bool = False) -> int:
+        d = make_desc(dims, vecs, inp=inp, out=inp if inplace else out, fft_type=fft_type, direction=direction,
+                      placement=0 if inplace else 1, ilp64=True)
+        return load().aoclfftz_gpu_server_submit(self.handle, ctypes.byref(d), 4 if precision == "f" else 8)
+
+    def test(self, rid: int) -> int:
+        return load().aoclfftz_gpu_server_test(self.handle, rid)
+
+    def wait(self, rid: int):
+        lat = ctypes.c_double(0.0)
+        rc = load().aoclfftz_gpu_server_wait(self.handle, rid, ctypes.byref(lat))
+        return rc, lat.value
+
+    def stats(self) -> dict:
+        buf = (ctypes.c_double * 16)()
+        n = load().aoclfftz_gpu_server_stats(self.handle, buf, 16)
+        return dict(zip(self.STATS, list(buf[:n])))
+
+    def reset_stats(self) -> None:
+        load().aoclfftz_gpu_server_reset_stats(self.handle)
+
+    def destroy(self) -> None:
+        if self.handle:
+            load().aoclfftz_gpu_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
 
 
 def batch_plan_json(items, precision: str = "f", direction: int = 0) -> dict:

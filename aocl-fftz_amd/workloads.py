@@ -72,3 +72,32 @@ def sinusoids(n: int, batch: int, seed: int = 4, tones: int = 8, noise: float = 
         x = x + noise * (rng.standard_normal(n) + 1j * rng.standard_normal(n))
         out[b] = x.astype(np.complex64)
     return out
+
+
+def serving_requests(count: int = 4000, seed: int = 11, max_elems: int = 1 << 21):
+    """Request mix for the serving loop: 50% 1-D C2C (N = 2^6 .. 2^20), 25% 2-D
+    R2C (128^2 .. 4096^2), 25% 3-D C2C (32^3 .. 256^3); 10% complex128; batch
+    a power of two in 1..256, reduced so one request holds at most max_elems
+    elements.  Each entry: (dims, vecs, precision, fft_type, in_elems,
+    out_elems, in_real)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        kind = int(rng.choice(3, p=[0.5, 0.25, 0.25]))
+        prec = "d" if rng.random() < 0.1 else "f"
+        b = 1 << int(rng.integers(0, 9))
+        if kind == 0:
+            n = 1 << int(rng.integers(6, 21))
+            b = max(1, min(b, max_elems // n))
+            out.append(([(n, 1, 1)], [(b, n, n)], prec, 0, n * b, n * b, False))
+        elif kind == 1:
+            n = 1 << int(rng.integers(7, 13))
+            h = n // 2 + 1
+            b = max(1, min(b, max_elems // (n * n)))
+            out.append(([(n, 1, 1), (n, n, h)], [(b, n * n, n * h)], prec, 1, n * n * b, n * h * b, True))
+        else:
+            n = 1 << int(rng.integers(5, 9))
+            b = max(1, min(b, max_elems // n ** 3))
+            out.append(([(n, 1, 1), (n, n, n), (n, n * n, n * n)], [(b, n ** 3, n ** 3)], prec, 0, n ** 3 * b,
+                        n ** 3 * b, False))
+    return out

@@ -17,6 +17,9 @@ Also measured in the same run and reported in the same JSON line:
   * the precision workload: batched complex64 N = 4096, batch 16384, lines
     of 8 tones + noise, plain FP32 against the matrix-core forms (FP32, bf16
     hi+lo, scaled fp16 hi+lo operands), time and error against complex128;
+  * serving: open-loop Poisson arrivals (2k, 10k, 50k requests/s) of a mix
+    of 1-D C2C, 2-D R2C and 3-D C2C requests through the serving runtime,
+    p50 / p99 latency and transforms/s;
   * roofline of the dominant kernel: algorithmic bytes per launch over its
     average duration measured with HIP events on its own stream;
   * the reference's CPU path (AOCL-FFTZ built from its sources, OpenMP) timed
@@ -345,6 +348,77 @@ def bench_split(torch, fftz, steps, warmup, n=4096, b=16384):
     return out
 
 
+def bench_serving(torch, fftz, rates=(2000, 10000, 50000), seconds=0.4, max_count=4000):
+    """Serving: open-loop Poisson arrivals of the serving request mix (1-D C2C,
+    2-D R2C, 3-D C2C, 10% complex128) through one server (8 slots, 16 GiB
+    budget); request buffers are carved from a ring in one device arena.
+    Plans are warmed first; p50 / p99 latency and throughput per rate."""
+    import numpy as np
+    import workloads
+    reqs = workloads.serving_requests(count=max_count)
+    arena_bytes = 4 << 30
+    arena = torch.empty(arena_bytes // 4, dtype=torch.float32, device="cuda").uniform_(-1.0, 1.0)
+    base = arena.data_ptr()
+    srv = fftz.Server(16 << 30, max_inflight=8)
+    ring = [0]
+
+    def carve(nbytes):
+        nbytes = (nbytes + 255) & ~255
+        if ring[0] + nbytes > arena_bytes:
+            ring[0] = 0
+        off = ring[0]
+        ring[0] += nbytes
+        return base + off
+
+    def submit(r):
+        dims, vecs, prec, ftype, nin, nout, in_real = r
+        esz = 8 if prec == "f" else 16
+        inp = carve(nin * (esz // 2 if in_real else esz))
+        out = carve(nout * esz)
+        return srv.submit(dims, vecs, prec, 0, fft_type=ftype, inp=inp, out=out)
+
+    seen = set()
+    for r in reqs:  # warm: one request per distinct problem builds its plan
+        key = (str(r[0]), str(r[1]), r[2], r[3])
+        if key not in seen:
+            seen.add(key)
+            srv.wait(submit(r))
+    torch.cuda.synchronize()
+    out = {"distinct_problems": len(seen), "requests_mean_MiB": round(float(np.mean(
+        [(r[4] * ((8 if r[2] == "f" else 16) // (2 if r[6] else 1)) + r[5] * (8 if r[2] == "f" else 16)) / 2 ** 20
+         for r in reqs])), 3)}
+    rng = np.random.default_rng(5)
+    for rate in rates:
+        count = int(min(max_count, rate * seconds))
+        gaps = rng.exponential(1.0 / rate, count)
+        srv.reset_stats()
+        ids = []
+        t0 = time.perf_counter()
+        t_next = t0
+        for i in range(count):
+            t_next += gaps[i]
+            while time.perf_counter() < t_next:
+                pass
+            ids.append(submit(reqs[i]))
+        t_sub = time.perf_counter() - t0
+        for rid in ids:
+            srv.wait(rid)
+        st = srv.stats()
+        out[f"rate_{rate}"] = {"requests": count, "offered_per_s": rate,
+                               "submitted_per_s": round(count / t_sub, 1),
+                               "p50_ms": round(st["p50_ms"], 4), "p99_ms": round(st["p99_ms"], 4),
+                               "max_ms": round(st["max_ms"], 4),
+                               "transforms_per_s": round(st["transforms_per_s"], 1),
+                               "gflops": round(st["gflops"], 1), "queue_peak": int(st["queue_peak"]),
+                               "failed": int(st["failed"])}
+        log(f"[bench] serving {rate}/s: p50 {st['p50_ms']:.3f} ms p99 {st['p99_ms']:.3f} ms "
+            f"{st['transforms_per_s']:.0f} transforms/s")
+    srv.destroy()
+    del arena
+    torch.cuda.empty_cache()
+    return out
+
+
 def copy_ceiling(torch, mib=256, reps=20):
     """Plain device-to-device copy of a known size: the practical HBM
     ceiling on this box, and the calibration kernel for FETCH/WRITE_SIZE."""
@@ -441,6 +515,8 @@ def main():
     log(f"[bench] uneven: {ru['ms_per_execute']:.3f} ms {ru['gflops']:.1f} GFLOP/s frac {ru['frac_of_hbm']:.3f}")
     log("[bench] start split-precision matrix-core vs plain, N=4096 batch 16384")
     rs = bench_split(torch, fftz, args.steps, args.warmup)
+    log("[bench] start serving (Poisson arrivals)")
+    rv = bench_serving(torch, fftz)
     copy_bps = copy_ceiling(torch)
     log(f"[bench] copy ceiling {copy_bps / 1e9:.0f} GB/s")
     base, extra = (None, {})
@@ -476,6 +552,7 @@ def main():
                                if not k.startswith("_")},
         "uneven_batch_fp32": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in ru.items()},
         "split_mfma_n4096_b16384": rs,
+        "serving": rv,
     }
     line["copy_ceiling_GBps"] = round(copy_bps / 1e9, 1)
     line.update(extra)

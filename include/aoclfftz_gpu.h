@@ -110,6 +110,48 @@ FFTZ_INT64 aoclfftz_gpu_batch_plan_json(const aoclfftz_gpu_batch_item *items, FF
                                         FFTZ_INT32 precision_bytes, FFTZ_INT32 direction, FFTZ_CHAR *buf,
                                         FFTZ_INT64 buflen);
 
+/* Serving runtime: independent transform requests sharing one GPU.
+ *
+ * A server keeps one plan (and its tables) resident per distinct problem and
+ * runs requests on up to max_inflight execution slots at once.  A request is
+ * admitted when a slot is free and its HBM footprint -- input and output
+ * extents plus the plan's work buffer, in bytes -- fits hbm_budget_bytes
+ * next to the requests already running; admission is first come, first
+ * served, and requests that arrive while others run are admitted as slots
+ * and memory free up.  A progress thread owned by the server does the
+ * admitting and completion tracking.  No counterpart in the reference (a
+ * CPU library executes one call at a time on the caller's thread). */
+
+/* NULL on bad arguments or when no device is visible. */
+FFTZ_VOID *aoclfftz_gpu_server_create(FFTZ_INT64 hbm_budget_bytes, FFTZ_INT32 max_inflight);
+
+/* Queue one request.  problem: an ILP64 descriptor (aoclfftz_prob_desc_f_64_
+ * or aoclfftz_prob_desc_d_64_, per precision_bytes 4 / 8) whose in / out are
+ * device buffers that stay valid until the request completes.  Returns the
+ * request id (>= 0), -1 for an invalid problem, -2 when the footprint alone
+ * exceeds the budget.  Thread-safe; returns without waiting. */
+FFTZ_INT64 aoclfftz_gpu_server_submit(FFTZ_VOID *server, FFTZ_VOID *problem, FFTZ_INT32 precision_bytes);
+
+/* 1 when request id has completed (or failed), 0 while queued or running,
+ * -1 for an unknown id. */
+FFTZ_INT32 aoclfftz_gpu_server_test(FFTZ_VOID *server, FFTZ_INT64 id);
+
+/* Block until request id completes; writes its latency (submit to
+ * completion, seconds) and forgets the id.  0 on success, -1 on failure or
+ * an unknown id. */
+FFTZ_INT32 aoclfftz_gpu_server_wait(FFTZ_VOID *server, FFTZ_INT64 id, FFTZ_DOUBLE *latency_s);
+
+/* Statistics since creation or the last reset, written to out[0..cap):
+ * completed, failed, p50 latency (ms), p99 latency (ms), max latency (ms),
+ * transforms/s, GFLOP/s (5 N log2 N), peak admitted footprint (bytes),
+ * plans resident, peak queue length, span (s, first submit to last
+ * completion).  Returns the number of values (11). */
+FFTZ_INT32 aoclfftz_gpu_server_stats(FFTZ_VOID *server, FFTZ_DOUBLE *out, FFTZ_INT32 cap);
+FFTZ_VOID aoclfftz_gpu_server_reset_stats(FFTZ_VOID *server);
+
+/* Drains queued and running requests, then frees plans and the server. */
+FFTZ_VOID aoclfftz_gpu_server_destroy(FFTZ_VOID *server);
+
 #ifdef __cplusplus
 }
 #endif
