@@ -490,6 +490,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--legs", default="uneven,split,serving",
+                    help="extra workloads after the primary and N=1024 legs (comma list; '' for none)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the transforms here "
@@ -510,13 +512,18 @@ def main():
     r10 = bench_batched_n1024(torch, fftz, args.steps, args.warmup)
     log(f"[bench] n1024: {r10['ms_per_execute'] * 1e3:.2f} us {r10['gflops']:.1f} GFLOP/s "
         f"frac {r10['frac_of_hbm']:.3f}")
-    log("[bench] start uneven batch (20,000 items) fp32")
-    ru = bench_uneven(torch, fftz, args.steps, args.warmup)
-    log(f"[bench] uneven: {ru['ms_per_execute']:.3f} ms {ru['gflops']:.1f} GFLOP/s frac {ru['frac_of_hbm']:.3f}")
-    log("[bench] start split-precision matrix-core vs plain, N=4096 batch 16384")
-    rs = bench_split(torch, fftz, args.steps, args.warmup)
-    log("[bench] start serving (Poisson arrivals)")
-    rv = bench_serving(torch, fftz)
+    legs = set(x for x in args.legs.split(",") if x)
+    ru = rs = rv = None
+    if "uneven" in legs:
+        log("[bench] start uneven batch (20,000 items) fp32")
+        ru = bench_uneven(torch, fftz, args.steps, args.warmup)
+        log(f"[bench] uneven: {ru['ms_per_execute']:.3f} ms {ru['gflops']:.1f} GFLOP/s frac {ru['frac_of_hbm']:.3f}")
+    if "split" in legs:
+        log("[bench] start split-precision matrix-core vs plain, N=4096 batch 16384")
+        rs = bench_split(torch, fftz, args.steps, args.warmup)
+    if "serving" in legs:
+        log("[bench] start serving (Poisson arrivals)")
+        rv = bench_serving(torch, fftz)
     copy_bps = copy_ceiling(torch)
     log(f"[bench] copy ceiling {copy_bps / 1e9:.0f} GB/s")
     base, extra = (None, {})
@@ -550,7 +557,7 @@ def main():
         "launch_ms_avg": r24["launch_ms_avg"],
         "batched_n1024_fp32": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r10.items()
                                if not k.startswith("_")},
-        "uneven_batch_fp32": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in ru.items()},
+        "uneven_batch_fp32": ru and {k: (round(v, 4) if isinstance(v, float) else v) for k, v in ru.items()},
         "split_mfma_n4096_b16384": rs,
         "serving": rv,
     }
