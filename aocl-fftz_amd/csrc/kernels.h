@@ -25,8 +25,9 @@ const void* kernel_symbol_reset();
 const void* kernel_symbol_fixed(int prec_bytes, int L);
 
 // Matrix-core pass (kernels_mfma.hip) for complex64, L = 256 or 4096, with
-// operand precision `mode` (MfmaMode); nullptr for other arguments.
+// operand precision `mode` (MfmaMode) and nb (2 or 4) column blocks per
+// wavefront (block = 1024 / nb threads); nullptr for other arguments.
 // Kernel arguments: (PassDesc, const float* fragments).
-const void* kernel_symbol_mfma(int mode, int L);
+const void* kernel_symbol_mfma(int mode, int L, int nb);
 
 }  // namespace fftz

@@ -109,6 +109,35 @@ __device__ __forceinline__ void load_tile(const PassDesc& p, const TileIo& io, c
         }
         return;
     }
+    if (p.flags & (kRealPromote | kHermIn)) {
+        // full-length real transforms: real x[e] as (x[e], 0), or the Hermitian
+        // extension of bins 0..n/2 (n = nvalid_in)
+        const int n = p.nvalid_in;
+        cplx<R> v[E];
+#pragma unroll
+        for (int k = 0; k < E; k++) {
+            const int e = t + k * p.TL;
+            v[k] = {R(0), R(0)};
+            if (e < n) {
+                if (p.flags & kRealPromote) {
+                    const R* srcr = reinterpret_cast<const R*>(io.in) + li.ioff;
+                    v[k].x = srcr[(int64_t)e * p.ies];
+                } else {
+                    const cplx<R>* srcc = reinterpret_cast<const cplx<R>*>(io.in) + li.ioff;
+                    const int m = 2 * e <= n ? e : n - e;
+                    v[k] = srcc[(int64_t)m * p.ies];
+                    if (2 * e > n) v[k].y = -v[k].y;
+                }
+                if (p.flags & kConjIn) v[k].y = -v[k].y;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < E; k++) {
+            const int e = t + k * p.TL;
+            if (e < p.L) line[lds_pad(e)] = v[k];
+        }
+        return;
+    }
     const cplx<R>* src = reinterpret_cast<const cplx<R>*>(io.in) + li.ioff;
     const int nv = p.nvalid_in;
     const int ncover = (p.flags & kC2RPre) ? p.L + 1 : p.L;
@@ -316,6 +345,15 @@ __device__ __forceinline__ void store_tile(const PassDesc& p, const TileIo& io, 
     if (p.flags & kConjOut) {
 #pragma unroll
         for (int k = 0; k < E; k++) v[k].y = -v[k].y;
+    }
+    if (p.flags & kRealPart) {
+        R* dst = reinterpret_cast<R*>(io.out) + li.ooff;
+#pragma unroll
+        for (int k = 0; k < E; k++) {
+            const int e = t + k * p.TL;
+            if (e < p.nvalid_out) dst[(int64_t)e * p.oes] = v[k].x;
+        }
+        return;
     }
     if (p.flags & kRealOut) {
         // C2R: element j is (x[2j], x[2j+1]); stored conjugated-domain value
@@ -581,7 +619,7 @@ template <typename R, bool ALLRAD, int E>
 __device__ __forceinline__ void pass_body(const PassDesc& p, const TileIo& io, cplx<R>* lds, int64_t l_base, int tid)
 {
     // (a zero-stage plan, L = 1, is a strided copy: the staged path below)
-    if (!(p.flags & (kRealIn | kC2RPre | kR2CPost)) && p.nst >= 1) {
+    if (!(p.flags & kStagedFlags) && p.nst >= 1) {
         pass_body_direct<R, ALLRAD, E>(p, io, lds, l_base, tid);
         return;
     }
@@ -696,7 +734,16 @@ __device__ __forceinline__ void bs_first(const PassDesc& p, cplx<R>* lds, int c,
                 const int e = i * nbl + j;
                 v[k][i] = {R(0), R(0)};
                 if (e < n) {
-                    cplx<R> x = src[(int64_t)e * p.ies];
+                    cplx<R> x;
+                    if (p.flags & kRealPromote) {
+                        x = {reinterpret_cast<const R*>(src)[(int64_t)e * p.ies], R(0)};
+                    } else if (p.flags & kHermIn) {
+                        const int m = 2 * e <= n ? e : n - e;
+                        x = src[(int64_t)m * p.ies];
+                        if (2 * e > n) x.y = -x.y;
+                    } else {
+                        x = src[(int64_t)e * p.ies];
+                    }
                     if (cj) x.y = -x.y;
                     v[k][i] = cmul(x, chirp[e]);
                 }
@@ -788,7 +835,10 @@ __device__ __forceinline__ void bs_final(const PassDesc& p, const cplx<R>* lds, 
                 if (e < n) {
                     cplx<R> y = cmul(cconj(chirp[e]), v[k][i]);  // B * W
                     if (fwd) y.y = -y.y;                          // forward result is conj(B W)
-                    dst[(int64_t)e * p.oes] = y;
+                    if (p.flags & kRealPart)
+                        reinterpret_cast<R*>(dst)[(int64_t)e * p.oes] = y.x;
+                    else
+                        dst[(int64_t)e * p.oes] = y;
                 }
             }
         }
@@ -805,8 +855,13 @@ __device__ __forceinline__ void bluestein_body_direct(const PassDesc& p, const T
     cplx<R>* dst = nullptr;
     if (live) {
         LineIdx li = line_index(p, io, (uint32_t)(l_base + c));
-        src = reinterpret_cast<const cplx<R>*>(io.in) + li.ioff;
-        dst = reinterpret_cast<cplx<R>*>(io.out) + li.ooff;
+        // real-side lines are offset in real elements (and then only read /
+        // written through real pointers)
+        src = (p.flags & kRealPromote)
+                  ? reinterpret_cast<const cplx<R>*>(reinterpret_cast<const R*>(io.in) + li.ioff)
+                  : reinterpret_cast<const cplx<R>*>(io.in) + li.ioff;
+        dst = (p.flags & kRealPart) ? reinterpret_cast<cplx<R>*>(reinterpret_cast<R*>(io.out) + li.ooff)
+                                    : reinterpret_cast<cplx<R>*>(io.out) + li.ooff;
     }
     const int S = p.nst;
     FFTZ_RADIX_SWITCH(p.radix[0], ALLRAD, (bs_first<R, E, RAD>(p, lds, c, t, src, live)))

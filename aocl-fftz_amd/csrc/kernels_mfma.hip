@@ -218,16 +218,15 @@ __device__ __forceinline__ void dft16_blocks(const float (&x)[NB][8], const Frag
 
 }  // namespace
 
-// One workgroup (4 wavefronts) per tile of C lines, C * L / 256 = 16 column
-// blocks per stage, 4 per wavefront.
-template <int MODE, int L>
-__global__ void __launch_bounds__(256) k_mfma_pass(const PassDesc p, const float* __restrict__ frag)
+// One workgroup per tile of C lines, C * L / 256 = 16 column blocks per stage,
+// NB per wavefront (16 / NB wavefronts per workgroup).
+template <int MODE, int L, int NB>
+__global__ void __launch_bounds__(64 * 16 / NB) k_mfma_pass(const PassDesc p, const float* __restrict__ frag)
 {
     constexpr int S = L == 256 ? 2 : 3;
     constexpr int nbl = L / 16;           // butterflies per line per stage
     constexpr int CPL = L / 256;          // column blocks per line
     constexpr int LP = ((L - 1) + ((L - 1) >> 4) + 1) | 1;
-    constexpr int NB = 4;                 // column blocks per wavefront
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     cplx<float>* lds = reinterpret_cast<cplx<float>*>(smem_raw);
     const int tid = threadIdx.x;
@@ -326,14 +325,26 @@ __global__ void __launch_bounds__(256) k_mfma_pass(const PassDesc p, const float
     }
 }
 
-const void* kernel_symbol_mfma(int mode, int L)
+template <int MODE, int NB>
+static const void* mfma_sym(int L)
 {
+    return L == 256 ? (const void*)&k_mfma_pass<MODE, 256, NB> : (const void*)&k_mfma_pass<MODE, 4096, NB>;
+}
+
+const void* kernel_symbol_mfma(int mode, int L, int nb)
+{
+    if (nb == 4) {
+        switch (mode) {
+            case kMfmaF32: return mfma_sym<kMfmaF32, 4>(L);
+            case kMfmaBf16x3: return mfma_sym<kMfmaBf16x3, 4>(L);
+            case kMfmaF16x2: return mfma_sym<kMfmaF16x2, 4>(L);
+            default: return nullptr;
+        }
+    }
     switch (mode) {
-        case kMfmaF32: return L == 256 ? (const void*)&k_mfma_pass<kMfmaF32, 256> : (const void*)&k_mfma_pass<kMfmaF32, 4096>;
-        case kMfmaBf16x3:
-            return L == 256 ? (const void*)&k_mfma_pass<kMfmaBf16x3, 256> : (const void*)&k_mfma_pass<kMfmaBf16x3, 4096>;
-        case kMfmaF16x2:
-            return L == 256 ? (const void*)&k_mfma_pass<kMfmaF16x2, 256> : (const void*)&k_mfma_pass<kMfmaF16x2, 4096>;
+        case kMfmaF32: return mfma_sym<kMfmaF32, 2>(L);
+        case kMfmaBf16x3: return mfma_sym<kMfmaBf16x3, 2>(L);
+        case kMfmaF16x2: return mfma_sym<kMfmaF16x2, 2>(L);
         default: return nullptr;
     }
 }

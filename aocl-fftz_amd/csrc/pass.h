@@ -36,7 +36,12 @@ enum PassFlags : uint32_t {
     kC2RPre = 1u << 8,     // real-packed separate before the L-point FFT
     kRealIn = 1u << 9,     // R2C: input is real, read as L complex pairs
     kRealOut = 1u << 10,   // C2R: output is real, written as L complex pairs
+    kRealPromote = 1u << 11,  // R2C (any length): real element j loaded as (x[j], 0)
+    kHermIn = 1u << 12,       // C2R (any length): bins 0..n/2 stored, bin k > n/2 is conj(bin n-k)
+    kRealPart = 1u << 13,     // C2R (any length): store only the real part, as real elements
 };
+// Flags whose passes stage the tile through LDS before the first stage.
+constexpr uint32_t kStagedFlags = kRealIn | kC2RPre | kR2CPost | kRealPromote | kHermIn | kRealPart;
 
 // Operand precision of the matrix-core radix-16 pass (kernels_mfma.hip).
 enum MfmaMode : int {

@@ -61,8 +61,9 @@ const DeviceInfo& device_info()
                                           (int)info.lds_per_block);
         for (int m : {kMfmaF32, kMfmaBf16x3, kMfmaF16x2})
             for (int L : {256, 4096})
-                (void)hipFuncSetAttribute(kernel_symbol_mfma(m, L), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                          (int)info.lds_per_block);
+                for (int nb : {2, 4})
+                    (void)hipFuncSetAttribute(kernel_symbol_mfma(m, L, nb), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                              (int)info.lds_per_block);
     });
     return info;
 }
@@ -415,7 +416,7 @@ static bool build_pass(Plan* plan, TableBuilder& tb, const PassReq& q, std::stri
     const bool pow2L = q.L >= 64 && q.L <= 4096 && (q.L & (q.L - 1)) == 0;
     const bool fixed_ok = q.kind == KernelKind::Pass && pow2L && E == 16 && d.nvalid_in == q.L &&
                           d.nvalid_out == q.L &&
-                          !(q.flags & (kRealIn | kRealOut | kR2CPost | kC2RPre)) && env_int("FFTZ_FIXED", 1);
+                          !(q.flags & (kStagedFlags | kRealOut)) && env_int("FFTZ_FIXED", 1);
     bool plan_match = fixed_ok;
     if (fixed_ok) {
         // the fixed kernels hard-code 16, 16, ..., 2^rem (kernels_fixed.hip)
@@ -429,7 +430,7 @@ static bool build_pass(Plan* plan, TableBuilder& tb, const PassReq& q, std::stri
     // plan, row mapping, no fused twiddle / real-packed step
     const int mm = plan->spec.mfma_mode;
     const bool use_mfma = mm != kMfmaNone && prec == 4 && plan_match && (q.L == 256 || q.L == 4096) &&
-                          !q.want_in_col && !q.want_out_col && !(q.flags & kTw4);
+                          !q.want_in_col && !q.want_out_col && !(q.flags & (kTw4 | kStagedFlags | kRealOut));
 
     // line axes: choose l0
     std::vector<LineAxis> ax = merge_axes(q.lines);
@@ -525,7 +526,8 @@ static bool build_pass(Plan* plan, TableBuilder& tb, const PassReq& q, std::stri
     ln.fixed = plan_match && ln.block <= 256 && !use_mfma;
     if (use_mfma) {
         ln.mfma = mm;
-        ln.block = 256;
+        ln.mfma_nb = env_int("FFTZ_MFMA_NB", 4) == 2 ? 2 : 4;
+        ln.block = (uint32_t)(1024 / ln.mfma_nb);
         ln.frag_off = mfma_fragments(tb, mm);
     }
     const int64_t tiles = (nl + C - 1) / C;
@@ -817,24 +819,56 @@ static bool plan_c2c(Plan* plan, TableBuilder& tb, std::vector<AxisSpec> dims, c
     return true;
 }
 
+// Full-length pass for a real axis of length n (odd n, or an even n whose half
+// has no radix plan): the n-point complex transform of the real line promoted
+// to complex (R2C), or of the Hermitian extension of the n/2+1 stored bins
+// with only the real part stored (C2R).  Bluestein when n has a prime factor
+// above 13.  Twice the arithmetic of the packed form; same HBM traffic.
+static bool real_full_pass(Plan* plan, TableBuilder& tb, PassReq q, int64_t n, std::string& err)
+{
+    const int prec = plan->spec.prec_bytes;
+    const bool r2c = (q.flags & kRealPromote) != 0;
+    q.nvalid_in = (int)n;
+    q.nvalid_out = r2c ? (int)(n / 2 + 1) : (int)n;
+    q.want_in_col = q.want_out_col = false;
+    if (smooth13(n)) {
+        if (n > single_pass_limit(prec)) {
+            err = "real length " + std::to_string(n) + " above the single-pass limit";
+            return false;
+        }
+        q.L = (int)n;
+        return build_pass(plan, tb, q, err);
+    }
+    const int64_t M = bluestein_length(n);
+    if (M > single_pass_limit(prec)) {
+        err = "Bluestein length " + std::to_string(M) + " for real length " + std::to_string(n) +
+              " exceeds the fused single-pass limit";
+        return false;
+    }
+    q.kind = KernelKind::Bluestein;
+    q.L = (int)M;
+    bluestein_tables(tb, prec, n, M, &q.chirp_off, &q.spec_off);
+    return build_pass(plan, tb, q, err);
+}
+
 static bool plan_real(Plan* plan, TableBuilder& tb, std::vector<AxisSpec> dims, const std::vector<AxisSpec>& vecs,
                       std::string& err)
 {
     const int prec = plan->spec.prec_bytes;
     const bool r2c = plan->spec.type == FftType::R2C;
     const int64_t n0 = dims[0].n;
-    if (n0 % 2 != 0) {
-        err = "real transforms need an even innermost length";
-        return false;
-    }
     const int64_t L = n0 / 2;
-    if (!smooth13(L) || L > single_pass_limit(prec)) {
-        err = "real-packed length " + std::to_string(L) + " unsupported";
-        return false;
+    // real-packed (n0/2-point complex transform + recombine / separate) when
+    // n0 is even and its half has a single-pass radix plan; otherwise the
+    // full-length promoted / Hermitian pass
+    const bool packed = n0 % 2 == 0 && smooth13(L) && L <= single_pass_limit(prec) && L >= 1;
+    const int64_t nbins = n0 / 2 + 1;
+    int64_t rtw_off = -1;
+    if (packed) {
+        std::vector<d2> rtw((size_t)(L / 2 + 1));
+        for (int64_t k = 0; k <= L / 2; k++) rtw[(size_t)k] = root_fwd(k, n0);
+        rtw_off = tb.add_complex(rtw, prec);
     }
-    std::vector<d2> rtw((size_t)(L / 2 + 1));
-    for (int64_t k = 0; k <= L / 2; k++) rtw[(size_t)k] = root_fwd(k, n0);
-    const int64_t rtw_off = tb.add_complex(rtw, prec);
     // complex geometry of the half-spectrum side
     std::vector<AxisSpec> cd = dims;  // n of dim 0 becomes L+1 bins on the complex side
     if (r2c) {
@@ -844,19 +878,24 @@ static bool plan_real(Plan* plan, TableBuilder& tb, std::vector<AxisSpec> dims, 
         q.oes = dims[0].os;
         for (size_t j = 1; j < dims.size(); j++) q.lines.push_back({dims[j].n, dims[j].is, dims[j].os});
         for (auto& v : vecs) q.lines.push_back({v.n, v.is, v.os});
-        q.flags = kRealIn | kR2CPost;
-        q.rtw_off = rtw_off;
         q.in_role = Role::In;
         q.out_role = Role::Out;
-        if (!build_pass(plan, tb, q, err)) return false;
+        if (packed) {
+            q.flags = kRealIn | kR2CPost;
+            q.rtw_off = rtw_off;
+            if (!build_pass(plan, tb, q, err)) return false;
+        } else {
+            q.flags = kRealPromote;
+            if (!real_full_pass(plan, tb, q, n0, err)) return false;
+        }
         // remaining axes: C2C forward in place on the output
-        cd[0].n = L + 1;
+        cd[0].n = nbins;
         for (size_t a = 1; a < dims.size(); a++) {
             std::vector<AxisSpec> od = cd;
             if (!plan_axis_c2c(plan, tb, od, vecs, a, false, false, false, Role::Out, Role::Out, true, err))
                 return false;
         }
-        plan->description += "real-packed R2C; ";
+        plan->description += packed ? "real-packed R2C; " : "real R2C (full length); ";
         return true;
     }
     // C2R: outer axes backward into the work buffer, then separate + FFT.
@@ -867,7 +906,7 @@ static bool plan_real(Plan* plan, TableBuilder& tb, std::vector<AxisSpec> dims, 
         // dense work layout: [vec][dims reversed], dim0 has L+1 bins
         int64_t stride = 1;
         std::vector<AxisSpec> dd = dims;
-        dd[0].n = L + 1;
+        dd[0].n = nbins;
         for (size_t j = 0; j < dd.size(); j++) {
             wdims[j] = {dd[j].n, dd[j].is, stride};
             stride *= dd[j].n;
@@ -893,16 +932,23 @@ static bool plan_real(Plan* plan, TableBuilder& tb, std::vector<AxisSpec> dims, 
     }
     PassReq q;
     q.L = (int)L;
-    q.nvalid_in = (int)(L + 1);
-    q.extra_lds_elems = 1;
     q.ies = wdims[0].is;
     q.oes = dims[0].os;  // real elements
     for (size_t j = 1; j < dims.size(); j++) q.lines.push_back({dims[j].n, wdims[j].is, dims[j].os});
     for (size_t v = 0; v < vecs.size(); v++) q.lines.push_back({vecs[v].n, wvecs[v].is, vecs[v].os});
-    q.flags = kC2RPre | kConjOut | kRealOut;
-    q.rtw_off = rtw_off;
     q.in_role = src;
     q.out_role = Role::Out;
+    if (!packed) {
+        // backward = conjugate in and out around the forward transform
+        q.flags = kHermIn | kConjIn | kConjOut | kRealPart;
+        if (!real_full_pass(plan, tb, q, n0, err)) return false;
+        plan->description += "real C2R (full length); ";
+        return true;
+    }
+    q.nvalid_in = (int)(L + 1);
+    q.extra_lds_elems = 1;
+    q.flags = kC2RPre | kConjOut | kRealOut;
+    q.rtw_off = rtw_off;
     if (!build_pass(plan, tb, q, err)) return false;
     plan->description += "real-packed C2R; ";
     return true;
@@ -1405,7 +1451,7 @@ static hipError_t launch_all(Plan* p, Slot* s, const void* in, void* out, hipStr
         d.out = role_ptr(ln.out_role) + ln.out_off_bytes;
         const float* frag = ln.frag;
         void* args[] = {&d, &frag};
-        const void* fn = ln.mfma ? kernel_symbol_mfma(ln.mfma, ln.d.L)
+        const void* fn = ln.mfma ? kernel_symbol_mfma(ln.mfma, ln.d.L, ln.mfma_nb)
                          : ln.fixed ? kernel_symbol_fixed(p->spec.prec_bytes, ln.d.L)
                                     : kernel_symbol(ln.kind, p->spec.prec_bytes, ln.all_radix, ln.E, ln.maxt);
         hipError_t e = hipLaunchKernel(fn, dim3(ln.grid),

@@ -58,6 +58,7 @@ struct Launch {
     int maxt = 256;    // block-size bound of the kernel variant
     bool fixed = false;  // fixed-length kernel (kernels_fixed.hip)
     int mfma = 0;        // MfmaMode of a matrix-core pass (kernels_mfma.hip)
+    int mfma_nb = 4;     // its column blocks per wavefront
     PassDesc d{};
     Role in_role = Role::In, out_role = Role::Out;
     int64_t in_off_bytes = 0, out_off_bytes = 0;

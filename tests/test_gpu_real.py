@@ -27,7 +27,14 @@ def r2c_dims(shape, inplace):
     return dims, si, so
 
 
-@pytest.mark.parametrize("shape", [(2,), (16,), (1000,), (4096,), (16384,), (8, 16), (64, 48), (12, 10, 20), (32, 32, 32)])
+# odd innermost lengths and even ones whose half has no radix plan take the
+# full-length path (real promoted to complex; Hermitian extension for C2R),
+# Bluestein for prime factors above 13
+FULL_LENGTH = [(1,), (7,), (9,), (101,), (202,), (2187,), (9, 15), (5, 7, 11), (4, 101)]
+
+
+@pytest.mark.parametrize("shape", [(2,), (16,), (1000,), (4096,), (16384,), (8, 16), (64, 48), (12, 10, 20),
+                                   (32, 32, 32)] + FULL_LENGTH)
 @pytest.mark.parametrize("prec", ["f", "d"])
 def test_r2c_c2r_out_of_place(torch_gpu, shape, prec):
     torch = torch_gpu
@@ -62,7 +69,7 @@ def test_r2c_c2r_out_of_place(torch_gpu, shape, prec):
     q.destroy()
 
 
-@pytest.mark.parametrize("shape", [(64,), (8, 30), (6, 10, 16)])
+@pytest.mark.parametrize("shape", [(64,), (8, 30), (6, 10, 16), (9,), (6, 10, 15), (3, 101)])
 def test_r2c_inplace(torch_gpu, shape):
     torch = torch_gpu
     dims, si, so = r2c_dims(shape, True)
