@@ -10,6 +10,7 @@
 #include <mutex>
 #include <sstream>
 #include <string>
+#include <vector>
 
 #include "../../include/aoclfftz.h"
 #include "../../include/aoclfftz_gpu.h"
@@ -441,6 +442,96 @@ aoclfftz_error_type aoclfftz_gpu_set_graph(FFTZ_VOID* handle, FFTZ_INT32 enable)
     h->plan->use_graph = enable != 0;
     return AOCLFFTZ_SUCCESS;
 }
+
+aoclfftz_error_type aoclfftz_gpu_autotune(FFTZ_VOID* handle, FFTZ_VOID* in, FFTZ_VOID* out, FFTZ_INT32 reps)
+{
+    Handle* h = as_handle(handle);
+    if (!h || !in || !out || reps <= 0) return AOCLFFTZ_EXECUTION_FAILURE;
+    if (!is_device_ptr(in) || !is_device_ptr(out)) {
+        report_error(__func__, "autotuning runs on device buffers");
+        return AOCLFFTZ_INVALID_INPUT;
+    }
+    std::lock_guard<std::mutex> g(h->amu);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    fftz::Plan* p = h->plan;
+    if (p->noop) return AOCLFFTZ_SUCCESS;
+    if (p->is_batch) {
+        // workgroups per CU of the small and medium tile classes
+        double best = 1e30;
+        int bs = 0, bm = 0;
+        for (int ws : {4, 8, 16})
+            for (int wm : {1, 2}) {
+                fftz::batch_set_grids(p, ws, wm);
+                const double t = fftz::plan_time(p, in, out, reps);
+                if (t >= 0 && t < best) {
+                    best = t;
+                    bs = ws;
+                    bm = wm;
+                }
+            }
+        if (!bs) return AOCLFFTZ_EXECUTION_FAILURE;
+        fftz::batch_set_grids(p, bs, bm);
+        fftz::TuneParams t;
+        t.wg_small = bs;
+        t.wg_medium = bm;
+        fftz::tune_store(p->tune_key, t);
+        return AOCLFFTZ_SUCCESS;
+    }
+    // single-size plans: row tile width, and column tile lines when the plan
+    // has column passes
+    bool has_col = false;
+    for (const auto& ln : p->launches) has_col = has_col || (ln.d.flags & (fftz::kInCol | fftz::kOutCol));
+    std::vector<fftz::TuneParams> cands;
+    for (int rt : {64, 128, 256, 512, 1024}) {
+        if (!has_col) {
+            fftz::TuneParams t;
+            t.row_t = rt;
+            cands.push_back(t);
+            continue;
+        }
+        for (int cc : {4, 8, 16, 32}) {
+            fftz::TuneParams t;
+            t.row_t = rt;
+            t.col_c = cc;
+            cands.push_back(t);
+        }
+    }
+    fftz::Plan* best_plan = nullptr;
+    double best = 1e30;
+    for (const fftz::TuneParams& t : cands) {
+        fftz::ProblemSpec spec = p->spec;
+        spec.tune = t;
+        std::string err;
+        fftz::Plan* q = fftz::plan_create(spec, err);
+        if (!q) continue;
+        const double ms = fftz::plan_time(q, in, out, reps);
+        if (ms >= 0 && ms < best) {
+            if (best_plan) fftz::plan_destroy(best_plan);
+            best_plan = q;
+            best = ms;
+        } else {
+            fftz::plan_destroy(q);
+        }
+    }
+    if (!best_plan) return AOCLFFTZ_EXECUTION_FAILURE;
+    fftz::ProblemSpec key_spec = p->spec;
+    key_spec.tune = fftz::TuneParams{};
+    fftz::tune_store(fftz::problem_key(key_spec), best_plan->spec.tune);
+    best_plan->setup_in = p->setup_in;
+    best_plan->setup_out = p->setup_out;
+    best_plan->use_graph = p->use_graph;
+    best_plan->profile = p->profile;
+    if (h->aslot) {
+        fftz::plan_release_slot(p, h->aslot);
+        h->aslot = nullptr;
+    }
+    fftz::plan_destroy(p);
+    h->plan = best_plan;
+    return AOCLFFTZ_SUCCESS;
+}
+
+FFTZ_INT32 aoclfftz_gpu_tune_export(const FFTZ_CHAR* path) { return fftz::tune_export(path); }
+FFTZ_INT32 aoclfftz_gpu_tune_import(const FFTZ_CHAR* path) { return fftz::tune_import(path); }
 
 aoclfftz_error_type aoclfftz_gpu_set_compute_mode(FFTZ_VOID* handle, FFTZ_INT32 mode)
 {

@@ -470,7 +470,7 @@ static bool build_pass(Plan* plan, TableBuilder& tb, const PassReq& q, std::stri
         d.TL = TL;
         d.tl_shift = 4;
     } else if (in_col || out_col) {
-        const int cmax = env_int("FFTZ_COL_C", prec == 8 ? 16 : 32);
+        const int cmax = plan->spec.tune.col_c ? plan->spec.tune.col_c : env_int("FFTZ_COL_C", prec == 8 ? 16 : 32);
         const size_t budget = std::min(lds_cap, (size_t)env_int("FFTZ_COL_LDS", 96 * 1024));
         while (C * 2 <= cmax && (size_t)(C * 2) * line_bytes <= budget && (n0 % (C * 2)) == 0 &&
                C * 2 * TL <= tmax)
@@ -478,7 +478,7 @@ static bool build_pass(Plan* plan, TableBuilder& tb, const PassReq& q, std::stri
         if (n0 % C != 0 || C < 2) { in_col = false; out_col = false; C = 1; }
     }
     if (!(in_col || out_col) && !use_mfma) {
-        const int tgt = env_int("FFTZ_ROW_T", 256);
+        const int tgt = plan->spec.tune.row_t ? plan->spec.tune.row_t : env_int("FFTZ_ROW_T", 256);
         const size_t budget = std::min(lds_cap, (size_t)env_int("FFTZ_ROW_LDS", 64 * 1024));
         int64_t nl_total = 1;
         for (auto& a : ax) nl_total *= a.n;
@@ -1032,7 +1032,8 @@ static bool batch_size_desc(TableBuilder& tb, int prec, int64_t n, int E, int ma
     return true;
 }
 
-Plan* batch_plan_create(const std::vector<BatchItem>& items, int prec, int dir, std::string& err, bool dry_run)
+Plan* batch_plan_create(const std::vector<BatchItem>& items, int prec, int dir, std::string& err, bool dry_run,
+                        const TuneParams* tune)
 {
     Plan* plan = new Plan();
     plan->spec.prec_bytes = prec;
@@ -1133,9 +1134,7 @@ Plan* batch_plan_create(const std::vector<BatchItem>& items, int prec, int dir, 
         for (const SizeInfo& si : sizes)
             if (si.cls == c) lds = std::max(lds, (size_t)si.C * si.d.LP * esize);
         bc.lds_bytes = (uint32_t)(lds + kBatchHdrBytes);
-        const int per_cu = c == 0 ? env_int("FFTZ_BATCH_WG_SMALL", 8)
-                           : c == 1 ? env_int("FFTZ_BATCH_WG_MEDIUM", 2) : env_int("FFTZ_BATCH_WG_BIG", 1);
-        bc.grid = (uint32_t)std::max<int64_t>(1, std::min<int64_t>(bc.ntiles, (int64_t)dev.cus * per_cu));
+        bc.grid = 1;
         if (bc.ntiles) {
             std::vector<BatchTile> v;
             v.reserve(tl.size());
@@ -1145,6 +1144,13 @@ Plan* batch_plan_create(const std::vector<BatchItem>& items, int prec, int dir, 
         plan->bclasses.push_back(bc);
     }
     std::ostringstream os;
+    {
+        plan->tune_key = batch_key(items, prec, dir);
+        TuneParams t;
+        if (tune) t = *tune;
+        else (void)tune_lookup(plan->tune_key, t);
+        batch_set_grids(plan, t.wg_small, t.wg_medium);
+    }
     static const char* kClassName[3] = {"small", "medium", "large"};
     os << "uneven batch: " << plan->n_transforms << " transforms, " << sizes.size() << " lengths; tiles";
     for (int c = 0; c < 3; c++)
@@ -1184,11 +1190,118 @@ Plan* batch_plan_create(const std::vector<BatchItem>& items, int prec, int dir, 
     return plan;
 }
 
+void batch_set_grids(Plan* p, int wg_small, int wg_medium)
+{
+    const DeviceInfo& dev = device_info();
+    for (size_t c = 0; c < p->bclasses.size(); c++) {
+        BatchClass& bc = p->bclasses[c];
+        const int per_cu = c == 0 ? (wg_small ? wg_small : env_int("FFTZ_BATCH_WG_SMALL", 8))
+                           : c == 1 ? (wg_medium ? wg_medium : env_int("FFTZ_BATCH_WG_MEDIUM", 2))
+                                    : env_int("FFTZ_BATCH_WG_BIG", 1);
+        bc.grid = (uint32_t)std::max<int64_t>(1, std::min<int64_t>(bc.ntiles, (int64_t)dev.cus * per_cu));
+    }
+    // captured graphs hold the old grids
+    for (Slot* s : p->slots) {
+        if (s->graph) (void)hipGraphExecDestroy(s->graph);
+        s->graph = nullptr;
+        s->graph_in = nullptr;
+        s->graph_out = nullptr;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Tuned-parameter cache
+// ---------------------------------------------------------------------------
+namespace {
+std::mutex g_tune_mu;
+std::map<std::string, TuneParams> g_tune;
+}  // namespace
+
+std::string problem_key(const ProblemSpec& s)
+{
+    std::ostringstream os;
+    os << "p" << s.prec_bytes << ':' << (int)s.type << ':' << s.dir << ':' << s.inplace << ':' << s.mfma_mode << '|';
+    for (const auto& a : s.dims) os << a.n << ',' << a.is << ',' << a.os << ';';
+    os << '|';
+    for (const auto& a : s.vecs) os << a.n << ',' << a.is << ',' << a.os << ';';
+    return os.str();
+}
+
+std::string batch_key(const std::vector<BatchItem>& items, int prec, int dir)
+{
+    uint64_t h = 1469598103934665603ull;  // FNV-1a over the item list
+    auto mix = [&](int64_t v) {
+        for (int b = 0; b < 8; b++) {
+            h ^= (uint64_t)((v >> (8 * b)) & 0xff);
+            h *= 1099511628211ull;
+        }
+    };
+    for (const BatchItem& it : items) {
+        mix(it.n);
+        mix(it.batch);
+        mix(it.in_off);
+        mix(it.out_off);
+        mix(it.dist);
+    }
+    std::ostringstream os;
+    os << "b" << prec << ':' << dir << ':' << items.size() << ':' << std::hex << h;
+    return os.str();
+}
+
+bool tune_lookup(const std::string& key, TuneParams& out)
+{
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    auto it = g_tune.find(key);
+    if (it == g_tune.end()) return false;
+    out = it->second;
+    return true;
+}
+
+void tune_store(const std::string& key, const TuneParams& t)
+{
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    g_tune[key] = t;
+}
+
+int tune_export(const char* path)
+{
+    FILE* f = path ? fopen(path, "w") : nullptr;
+    if (!f) return -1;
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    fprintf(f, "# fftz tuned tile shapes: key row_t col_c wg_small wg_medium\n");
+    for (const auto& kv : g_tune)
+        fprintf(f, "%s %d %d %d %d\n", kv.first.c_str(), kv.second.row_t, kv.second.col_c, kv.second.wg_small,
+                kv.second.wg_medium);
+    const int n = (int)g_tune.size();
+    fclose(f);
+    return n;
+}
+
+int tune_import(const char* path)
+{
+    FILE* f = path ? fopen(path, "r") : nullptr;
+    if (!f) return -1;
+    char line[4096];
+    int n = 0;
+    while (fgets(line, sizeof(line), f)) {
+        if (line[0] == '#') continue;
+        char key[4096];
+        TuneParams t;
+        if (sscanf(line, "%4095s %d %d %d %d", key, &t.row_t, &t.col_c, &t.wg_small, &t.wg_medium) == 5) {
+            tune_store(key, t);
+            n++;
+        }
+    }
+    fclose(f);
+    return n;
+}
+
 Plan* plan_create(const ProblemSpec& spec_in, std::string& err, bool dry_run)
 {
     Plan* plan = new Plan();
     plan->spec = spec_in;
     ProblemSpec& spec = plan->spec;
+    if (spec.tune.empty()) (void)tune_lookup(problem_key(spec), spec.tune);
     if (!dry_run) {
         (void)hipGetDevice(&plan->device);
         (void)device_info();
@@ -1513,6 +1626,33 @@ hipError_t plan_enqueue(Plan* p, Slot* s, const void* in, void* out, hipStream_t
         s->graph_out = out;
     }
     return hipGraphLaunch(s->graph, stream);
+}
+
+}  // namespace fftz
+
+namespace fftz {
+
+double plan_time(Plan* p, const void* in, void* out, int reps)
+{
+    hipError_t e = hipSuccess;
+    Slot* s = plan_acquire_slot(p, &e);
+    if (!s) return -1.0;
+    hipEvent_t a = nullptr, b = nullptr;
+    double ms = -1.0;
+    if (hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess) {
+        e = plan_enqueue(p, s, in, out, s->stream);  // warm (and graph capture)
+        if (e == hipSuccess) e = hipEventRecord(a, s->stream);
+        for (int r = 0; r < reps && e == hipSuccess; r++) e = plan_enqueue(p, s, in, out, s->stream);
+        if (e == hipSuccess) e = hipEventRecord(b, s->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(b);
+        float t = 0.f;
+        if (e == hipSuccess && hipEventElapsedTime(&t, a, b) == hipSuccess) ms = (double)t / reps;
+    }
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    if (e != hipSuccess) (void)hipGetLastError();
+    plan_release_slot(p, s);
+    return ms;
 }
 
 }  // namespace fftz

@@ -26,6 +26,15 @@ struct AxisSpec {
 
 enum class FftType : int { C2C = 0, R2C = 1, C2R = 2 };
 
+// Tile-shape choices the autotuner varies; 0 keeps the planner's default.
+struct TuneParams {
+    int row_t = 0;      // threads per row tile (lines per tile = row_t / threads per line)
+    int col_c = 0;      // most lines per column tile
+    int wg_small = 0;   // uneven batch: workgroups per CU of the small class
+    int wg_medium = 0;  // uneven batch: workgroups per CU of the medium class
+    bool empty() const { return !row_t && !col_c && !wg_small && !wg_medium; }
+};
+
 struct ProblemSpec {
     int prec_bytes = 4;  // 4: float, 8: double
     FftType type = FftType::C2C;
@@ -34,6 +43,7 @@ struct ProblemSpec {
     std::vector<AxisSpec> dims;  // dims[0] innermost
     std::vector<AxisSpec> vecs;  // batch axes
     int mfma_mode = 0;           // MfmaMode for complex64 power-of-16 passes
+    TuneParams tune;             // empty: the tuned cache, else the defaults
 };
 
 // One uneven-batch entry: `batch` transforms of length n, `dist` elements
@@ -115,6 +125,7 @@ struct Plan {
     int64_t descs_off = -1;
     std::vector<BatchClass> bclasses;
     int64_t n_transforms = 0;
+    std::string tune_key;               // uneven batch: its key in the tuned cache
     double flops = 0.0;                 // 5 N log2 N per transform x batch (C2C)
 };
 
@@ -125,7 +136,23 @@ Plan* plan_create(const ProblemSpec& spec, std::string& err, bool dry_run = fals
 
 // Uneven batch of 1-D C2C transforms in one persistent launch per tile class.
 Plan* batch_plan_create(const std::vector<BatchItem>& items, int prec_bytes, int dir, std::string& err,
-                        bool dry_run = false);
+                        bool dry_run = false, const TuneParams* tune = nullptr);
+// Re-derive the uneven-batch grids for per-CU workgroup counts (0: default).
+void batch_set_grids(Plan* p, int wg_small, int wg_medium);
+
+// Process-wide cache of tuned tile shapes, keyed by problem (precision, type,
+// direction, placement, compute mode, dims, vecs) or by uneven-batch item list.
+// plan_create / batch_plan_create consult it when the spec carries no tuning.
+std::string problem_key(const ProblemSpec& spec);
+std::string batch_key(const std::vector<BatchItem>& items, int prec_bytes, int dir);
+bool tune_lookup(const std::string& key, TuneParams& out);
+void tune_store(const std::string& key, const TuneParams& t);
+int tune_export(const char* path);  // entries written, -1 on error
+int tune_import(const char* path);  // entries read, -1 on error
+
+// Kernel time of `reps` executes of p on (in, out) (device buffers) on a slot
+// stream, in ms; negative on a HIP error.
+double plan_time(Plan* p, const void* in, void* out, int reps);
 
 // JSON description of every launch of a plan (geometry, strides, flags).
 std::string plan_json(const Plan* p);

@@ -66,7 +66,8 @@ EXPORTS = (
     "aoclfftz_setup_f", "aoclfftz_setup_d", "aoclfftz_setup_f_64_", "aoclfftz_setup_d_64_",
     "aoclfftz_execute", "aoclfftz_execute_io", "aoclfftz_destroy", "aoclfftz_version",
     "aoclfftz_gpu_set_stream", "aoclfftz_gpu_synchronize", "aoclfftz_gpu_set_graph",
-    "aoclfftz_gpu_set_compute_mode",
+    "aoclfftz_gpu_set_compute_mode", "aoclfftz_gpu_autotune", "aoclfftz_gpu_tune_export",
+    "aoclfftz_gpu_tune_import",
     "aoclfftz_gpu_server_create", "aoclfftz_gpu_server_submit", "aoclfftz_gpu_server_test",
     "aoclfftz_gpu_server_wait", "aoclfftz_gpu_server_stats", "aoclfftz_gpu_server_reset_stats",
     "aoclfftz_gpu_server_destroy",
@@ -113,6 +114,12 @@ def load():
     lib.aoclfftz_gpu_set_graph.restype = ctypes.c_int
     lib.aoclfftz_gpu_set_compute_mode.argtypes = [vp, ctypes.c_int32]
     lib.aoclfftz_gpu_set_compute_mode.restype = ctypes.c_int
+    lib.aoclfftz_gpu_autotune.argtypes = [vp, vp, vp, ctypes.c_int32]
+    lib.aoclfftz_gpu_autotune.restype = ctypes.c_int
+    lib.aoclfftz_gpu_tune_export.argtypes = [ctypes.c_char_p]
+    lib.aoclfftz_gpu_tune_export.restype = ctypes.c_int32
+    lib.aoclfftz_gpu_tune_import.argtypes = [ctypes.c_char_p]
+    lib.aoclfftz_gpu_tune_import.restype = ctypes.c_int32
     lib.aoclfftz_gpu_server_create.argtypes = [ctypes.c_int64, ctypes.c_int32]
     lib.aoclfftz_gpu_server_create.restype = vp
     lib.aoclfftz_gpu_server_submit.argtypes = [vp, vp, ctypes.c_int32]
@@ -243,6 +250,11 @@ class Plan:
     def set_compute_mode(self, mode: int) -> int:
         return load().aoclfftz_gpu_set_compute_mode(self.handle, int(mode))
 
+    def autotune(self, inp, out, reps: int = 5) -> int:
+        """Time candidate tile shapes on device buffers (overwritten) and keep
+        the fastest; the choice is cached for later plans of this problem."""
+        return load().aoclfftz_gpu_autotune(self.handle, _ptr(inp), _ptr(out), int(reps))
+
     def profile(self, enable: bool) -> int:
         return load().aoclfftz_gpu_profile(self.handle, 1 if enable else 0)
 
@@ -371,6 +383,14 @@ def plan_json(dims, vecs, precision: str = "f", direction: int = 0, fft_type: in
     buf = ctypes.create_string_buffer(int(n) + 1)
     lib.aoclfftz_gpu_plan_json(ctypes.byref(d), pb, 1, buf, n + 1)
     return json.loads(buf.value.decode())
+
+
+def tune_export(path: str) -> int:
+    return load().aoclfftz_gpu_tune_export(path.encode())
+
+
+def tune_import(path: str) -> int:
+    return load().aoclfftz_gpu_tune_import(path.encode())
 
 
 def version() -> str:

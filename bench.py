@@ -238,8 +238,12 @@ def bench_uneven(torch, fftz, steps, warmup):
     t_plan = time.perf_counter()
     p = fftz.BatchPlan(items, "f", 0)
     t_plan = time.perf_counter() - t_plan
+    # tile classes' workgroups per CU chosen by the autotuner on these buffers
+    t_tune = time.perf_counter()
+    assert p.autotune(x, y, reps=2) == 0
+    t_tune = time.perf_counter() - t_tune
     assert p.set_stream(stream) == 0
-    log(f"[bench] uneven plan ({t_plan * 1e3:.0f} ms): {p.info}")
+    log(f"[bench] uneven plan ({t_plan * 1e3:.0f} ms, tuned in {t_tune * 1e3:.0f} ms): {p.info}")
     for _ in range(max(warmup, 1)):
         assert p.execute_io(x, y) == 0
     torch.cuda.synchronize()
@@ -269,6 +273,7 @@ def bench_uneven(torch, fftz, steps, warmup):
         "frac_of_hbm": byts / (ms * 1e-3) / HBM_PEAK,
         "class_ms": [round(v, 3) for v in prof[-1]] if prof else None,
         "plan_ms": t_plan * 1e3,
+        "autotune_ms": t_tune * 1e3,
         "plan": p.info,
     }
     p.destroy()

@@ -50,6 +50,22 @@ aoclfftz_error_type aoclfftz_gpu_set_graph(FFTZ_VOID *handle, FFTZ_INT32 enable)
  * of the plan qualifies.  Not to be called while the handle executes. */
 aoclfftz_error_type aoclfftz_gpu_set_compute_mode(FFTZ_VOID *handle, FFTZ_INT32 mode);
 
+/* Autotuner.  Times the handle's candidate tile shapes on the given device
+ * buffers (reps executes each; in / out are overwritten) and keeps the
+ * fastest: row-tile width and column-tile lines for an ordinary plan,
+ * workgroups per CU of each tile class for an uneven batch.  The choice is
+ * also stored in a process-wide cache keyed by the problem, which later
+ * setups of the same problem use without tuning again.  No counterpart in
+ * the reference (its plan choice is a cost model, selector/selector.c). */
+aoclfftz_error_type aoclfftz_gpu_autotune(FFTZ_VOID *handle, FFTZ_VOID *in, FFTZ_VOID *out, FFTZ_INT32 reps);
+
+/* Write / read the tuned cache as text (one problem per line).  Return the
+ * number of entries, or -1 when the file cannot be opened.  The counterpart
+ * of FFTW wisdom files for this library (the reference's wisdom calls are
+ * stubs). */
+FFTZ_INT32 aoclfftz_gpu_tune_export(const FFTZ_CHAR *path);
+FFTZ_INT32 aoclfftz_gpu_tune_import(const FFTZ_CHAR *path);
+
 /* Human-readable plan summary (pass list with lengths, radices, tiles).  The
  * string lives as long as the handle. */
 const FFTZ_CHAR *aoclfftz_gpu_plan_info(FFTZ_VOID *handle);

@@ -114,3 +114,20 @@ def test_uneven_full_size(torch_gpu):
     assert rt <= 2 * 2e-7 * 14
     fwd.destroy()
     bwd.destroy()
+
+
+def test_uneven_autotune(torch_gpu):
+    torch = torch_gpu
+    items, total = workloads.uneven_batch(300, seed=21, max_batch=8)
+    x = rand_dev(torch, total, 5, torch.complex64)
+    y = torch.empty_like(x)
+    p = fftz.BatchPlan(items, "f", 0)
+    assert p.autotune(x.clone(), torch.empty_like(x), reps=2) == 0
+    assert p.execute_io(x, y) == 0
+    torch.cuda.synchronize()
+    xh, yh = x.cpu().numpy(), y.cpu().numpy()
+    for n, b, io, oo in items[:40]:
+        ref = O.fft(xh[io:io + n * b].reshape(b, n).astype(np.complex128))
+        assert O.rel_l2(yh[oo:oo + n * b], ref.reshape(-1)) <= tol(n, "f"), (n, b)
+    print(p.info)
+    p.destroy()

@@ -253,3 +253,24 @@ def test_large_fourstep(torch_gpu, n, prec):
     y, info = run_c2c(torch_gpu, x)
     assert "four-step" in info
     check(y, x, prec)
+
+
+def test_autotune_keeps_results(torch_gpu, tmp_path):
+    """Autotuning picks a tile shape by timing candidates on the caller's
+    buffers; the tuned plan's result matches the oracle and the choice lands
+    in the exported cache."""
+    torch = torch_gpu
+    for n, b in ((1024, 512), (1 << 20, 2)):
+        x = rand_c((b, n), 9, "f")
+        dx = to_dev(torch, x)
+        dy = torch.empty_like(dx)
+        scratch = torch.empty_like(dx)
+        p = fftz.Plan([(n, 1, 1)], [(b, n, n)], "f", 0, inp=dx, out=dy)
+        assert p.autotune(dx.clone(), scratch, reps=3) == 0
+        assert p.execute() == 0
+        torch.cuda.synchronize()
+        assert O.rel_l2(dy.cpu().numpy(), O.fft(x.astype(np.complex128)).reshape(-1)) <= tol(n, "f")
+        print(f"n={n}: {p.info[:160]}")
+        p.destroy()
+    path = str(tmp_path / "tune.txt")
+    assert fftz.tune_export(path) >= 2
