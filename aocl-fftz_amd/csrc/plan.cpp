@@ -533,7 +533,10 @@ static bool build_pass(Plan* plan, TableBuilder& tb, const PassReq& q, std::stri
     const int64_t tiles = (nl + C - 1) / C;
     ln.grid = (uint32_t)tiles;
     ln.lds_bytes = (uint32_t)((size_t)C * line_bytes);
-    if ((in_col || out_col) && tiles >= 64 && env_int("FFTZ_XCD_REMAP", 1)) d.flags |= kXcdRemap;
+    // XCD-contiguous tile order: measured to help passes that read rows and
+    // store columns (the four-step's transposing last pass) and to cost ~5%
+    // on passes that read columns
+    if (out_col && !in_col && tiles >= 64 && env_int("FFTZ_XCD_REMAP", 1)) d.flags |= kXcdRemap;
 
     // expand host-looped axes
     std::vector<int64_t> ioffs{0}, ooffs{0};

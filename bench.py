@@ -12,6 +12,8 @@ Also measured in the same run and reported in the same JSON line:
   * batched FP32 C2C, N = 1024, batch 4096, out of place (single LDS pass),
     with input buffers rotated over 512 MiB so the 256 MiB Infinity Cache does
     not serve them, plus the cache-warm figure;
+  * the mixed-radix large transform, complex128 N = 2^20 * 3 round trip;
+  * 3-D R2C float32 512^3 with its error against float64;
   * the uneven batch: 20,000 complex64 items of mixed length (7-smooth and
     prime) with batch 1..64 each, in one persistent launch per tile class;
   * the precision workload: batched complex64 N = 4096, batch 16384, lines
@@ -218,6 +220,92 @@ def bench_batched_n1024(torch, fftz, steps, warmup, n=1024, b=4096):
     res["_y"] = ys[0].cpu().numpy()
     p.destroy()
     del xs, ys
+    torch.cuda.empty_cache()
+    return res
+
+
+def bench_mixed_radix(torch, fftz, steps, warmup, n=3 << 20):
+    """Mixed radix: 1-D C2C complex128 N = 2^20 * 3, in place, forward +
+    backward per step; round-trip error after the 1/N scale."""
+    g = torch.Generator(device="cuda")
+    g.manual_seed(2)
+    x = torch.randn(n, dtype=torch.complex128, device="cuda", generator=g)
+    x0 = x.clone()
+    stream = torch.cuda.current_stream()
+    fwd = fftz.Plan([(n, 1, 1)], [(1, n, n)], "d", 0, inplace=True, inp=x, out=x)
+    bwd = fftz.Plan([(n, 1, 1)], [(1, n, n)], "d", 1, inplace=True, inp=x, out=x)
+    for p in (fwd, bwd):
+        assert p.set_stream(stream) == 0
+    for _ in range(max(warmup, 1)):
+        fwd.execute()
+        bwd.execute()
+        x.mul_(1.0 / n)
+    x.copy_(x0)
+    torch.cuda.synchronize()
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    t0.record(stream)
+    for _ in range(steps):
+        fwd.execute()
+        bwd.execute()
+    t1.record(stream)
+    torch.cuda.synchronize()
+    ms = t0.elapsed_time(t1) / steps
+    # one more clean round trip for the error
+    x.copy_(x0)
+    fwd.execute()
+    bwd.execute()
+    torch.cuda.synchronize()
+    rt = (torch.linalg.vector_norm(x / n - x0) / torch.linalg.vector_norm(x0)).item()
+    res = {"n": n, "ms_per_step": round(ms, 4), "gflops": round(2 * 5.0 * n * math.log2(n) / (ms * 1e-3) / 1e9, 1),
+           "transform_frac_of_hbm": round(2 * 2 * 16.0 * n / (ms * 1e-3) / HBM_PEAK, 4),
+           "roundtrip_rel_l2": rt, "plan": fwd.info}
+    fwd.destroy()
+    bwd.destroy()
+    del x, x0
+    torch.cuda.empty_cache()
+    return res
+
+
+def bench_r2c_512(torch, fftz, steps, warmup, n=512):
+    """3-D R2C float32 512^3 -> complex64 512 x 512 x 257 (real-packed row pass
+    + two column passes); error against the library's float64 transform of
+    the same input."""
+    g = torch.Generator(device="cuda")
+    g.manual_seed(4)
+    x = torch.randn((n, n, n), dtype=torch.float32, device="cuda", generator=g)
+    h = n // 2 + 1
+    y = torch.empty((n, n, h), dtype=torch.complex64, device="cuda")
+    dims = [(n, 1, 1), (n, n, h), (n, n * n, n * h)]
+    vecs = [(1, n ** 3, n * n * h)]
+    p = fftz.Plan(dims, vecs, "f", 0, fft_type=1, inp=x, out=y)
+    stream = torch.cuda.current_stream()
+    p.set_stream(stream)
+    for _ in range(max(warmup, 1)):
+        p.execute_io(x, y)
+    torch.cuda.synchronize()
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    t0.record(stream)
+    for _ in range(steps):
+        p.execute_io(x, y)
+    t1.record(stream)
+    torch.cuda.synchronize()
+    ms = t0.elapsed_time(t1) / steps
+    info = p.info
+    p.destroy()
+    xd = x.double()
+    yd = torch.empty((n, n, h), dtype=torch.complex128, device="cuda")
+    q = fftz.Plan(dims, vecs, "d", 0, fft_type=1, inp=xd, out=yd)
+    q.set_stream(stream)
+    q.execute_io(xd, yd)
+    torch.cuda.synchronize()
+    q.destroy()
+    err = (torch.linalg.vector_norm(y.to(torch.complex128) - yd) / torch.linalg.vector_norm(yd)).item()
+    byts = 4.0 * n ** 3 + 8.0 * n * n * h
+    res = {"ms_per_execute": round(ms, 4), "gflops": round(2.5 * n ** 3 * math.log2(n ** 3) / (ms * 1e-3) / 1e9, 1),
+           "algorithmic_frac_of_hbm": round(byts / (ms * 1e-3) / HBM_PEAK, 4), "rel_l2_vs_fp64": err, "plan": info}
+    del x, y, xd, yd
     torch.cuda.empty_cache()
     return res
 
@@ -495,7 +583,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--legs", default="uneven,split,serving",
+    ap.add_argument("--legs", default="mixed,r2c,uneven,split,serving",
                     help="extra workloads after the primary and N=1024 legs (comma list; '' for none)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -518,7 +606,15 @@ def main():
     log(f"[bench] n1024: {r10['ms_per_execute'] * 1e3:.2f} us {r10['gflops']:.1f} GFLOP/s "
         f"frac {r10['frac_of_hbm']:.3f}")
     legs = set(x for x in args.legs.split(",") if x)
-    ru = rs = rv = None
+    ru = rs = rv = rm = rr = None
+    if "mixed" in legs:
+        log("[bench] start 2^20*3 fp64 round trip")
+        rm = bench_mixed_radix(torch, fftz, args.steps, args.warmup)
+        log(f"[bench] 2^20*3: {rm['ms_per_step']:.3f} ms/step {rm['gflops']:.1f} GFLOP/s rt {rm['roundtrip_rel_l2']:.2e}")
+    if "r2c" in legs:
+        log("[bench] start 512^3 R2C fp32")
+        rr = bench_r2c_512(torch, fftz, args.steps, args.warmup)
+        log(f"[bench] r2c 512^3: {rr['ms_per_execute']:.3f} ms {rr['gflops']:.1f} GFLOP/s err {rr['rel_l2_vs_fp64']:.2e}")
     if "uneven" in legs:
         log("[bench] start uneven batch (20,000 items) fp32")
         ru = bench_uneven(torch, fftz, args.steps, args.warmup)
@@ -563,6 +659,8 @@ def main():
         "batched_n1024_fp32": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r10.items()
                                if not k.startswith("_")},
         "uneven_batch_fp32": ru and {k: (round(v, 4) if isinstance(v, float) else v) for k, v in ru.items()},
+        "mixed_radix_fp64_n3x2p20": rm,
+        "r2c_fp32_512cubed": rr,
         "split_mfma_n4096_b16384": rs,
         "serving": rv,
     }
