@@ -1,4 +1,4 @@
-// Kernel symbols exported from kernels.hip to the host planner.
+// Kernel symbols exported from the kernel units to the host planner.
 #pragma once
 
 #include "pass.h"
@@ -15,10 +15,17 @@ enum class KernelKind { Pass, Bluestein };
 // 1024); the 256 variants get a larger VGPR budget.
 const void* kernel_symbol(KernelKind kind, int prec_bytes, bool all_radix, int E, int maxt);
 
-// Uneven-batch persistent kernel (kernels.hip::k_batch) and the counter
-// reset kernel launched before it.
+// Uneven-batch persistent kernel (k_batch) and the counter reset kernel
+// launched before it.  Classes: E = 8 with 256 threads, E = 16 with 512 or
+// 1024 threads.
 const void* kernel_symbol_batch(int prec_bytes, int E, int maxt);
 const void* kernel_symbol_reset();
+
+// Per-precision instantiation units behind kernel_symbol / kernel_symbol_batch.
+const void* kernel_symbol_pass_f32(KernelKind kind, bool all_radix, int E, int maxt);
+const void* kernel_symbol_pass_f64(KernelKind kind, bool all_radix, int E, int maxt);
+const void* kernel_symbol_batch_f32(int E, int maxt);
+const void* kernel_symbol_batch_f64(int E, int maxt);
 
 // Fixed-length pass kernel for power-of-two L in [64, 4096] (radix plan 16s
 // then remainder, 16 elements per thread, block <= 256); nullptr otherwise.

@@ -1,7 +1,7 @@
 // Fixed-length pass kernels: power-of-two L in [64, 4096] with the radix plan,
 // threads per line (TL = L / 16) and LDS pitch known at compile time.
 //
-// Differences from the generic pass (kernels.hip):
+// Differences from the generic pass (kernels_body.h):
 //   * stage 1 consumes the registers loaded from HBM: the load pattern
 //     e = t + k*TL is exactly the first radix-16 stage's operand set
 //     x[i*(L/16) + j], so the tile is not staged through LDS first;

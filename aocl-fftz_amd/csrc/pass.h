@@ -15,7 +15,7 @@
 //   Bluestein                                          -> 1 fused pass
 //   N-D (row-column)                                   -> 1 pass per axis
 //
-// Shared by host (plan.cpp) and device (kernels.hip): plain POD only.
+// Shared by host (plan.cpp) and device (kernels_body.h etc.): plain POD only.
 #pragma once
 
 #include <stdint.h>
