@@ -441,9 +441,9 @@ def bench_split(torch, fftz, steps, warmup, n=4096, b=16384):
     return out
 
 
-def bench_serving(torch, fftz, rates=(2000, 10000, 50000), seconds=0.4, max_count=4000):
+def bench_serving(torch, fftz, rates=(2000, 10000, 50000), seconds=0.4, max_count=4000, inflight=16):
     """Serving: open-loop Poisson arrivals of the serving request mix (1-D C2C,
-    2-D R2C, 3-D C2C, 10% complex128) through one server (8 slots, 16 GiB
+    2-D R2C, 3-D C2C, 10% complex128) through one server (16 slots, 16 GiB
     budget); request buffers are carved from a ring in one device arena.
     Plans are warmed first; p50 / p99 latency and throughput per rate."""
     import numpy as np
@@ -452,7 +452,7 @@ def bench_serving(torch, fftz, rates=(2000, 10000, 50000), seconds=0.4, max_coun
     arena_bytes = 4 << 30
     arena = torch.empty(arena_bytes // 4, dtype=torch.float32, device="cuda").uniform_(-1.0, 1.0)
     base = arena.data_ptr()
-    srv = fftz.Server(16 << 30, max_inflight=8)
+    srv = fftz.Server(16 << 30, max_inflight=inflight)
     ring = [0]
 
     def carve(nbytes):
@@ -477,7 +477,7 @@ def bench_serving(torch, fftz, rates=(2000, 10000, 50000), seconds=0.4, max_coun
             seen.add(key)
             srv.wait(submit(r))
     torch.cuda.synchronize()
-    out = {"distinct_problems": len(seen), "requests_mean_MiB": round(float(np.mean(
+    out = {"max_inflight": inflight, "distinct_problems": len(seen), "requests_mean_MiB": round(float(np.mean(
         [(r[4] * ((8 if r[2] == "f" else 16) // (2 if r[6] else 1)) + r[5] * (8 if r[2] == "f" else 16)) / 2 ** 20
          for r in reqs])), 3)}
     rng = np.random.default_rng(5)
