@@ -451,6 +451,10 @@ aoclfftz_error_type aoclfftz_gpu_autotune(FFTZ_VOID* handle, FFTZ_VOID* in, FFTZ
         report_error(__func__, "autotuning runs on device buffers");
         return AOCLFFTZ_INVALID_INPUT;
     }
+    if (h->plan->spec.inplace && in != out) {
+        report_error(__func__, "an in-place handle is tuned on one buffer (in == out)");
+        return AOCLFFTZ_INVALID_INPUT;
+    }
     std::lock_guard<std::mutex> g(h->amu);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     fftz::Plan* p = h->plan;

@@ -361,7 +361,10 @@ FFTZ_VOID aoclfftz_gpu_server_destroy(FFTZ_VOID* server)
         s->cv_work.notify_all();
     }
     s->progress.join();  // drains queued and running requests first
-    for (auto& kv : s->reqs) delete kv.second;
+    for (auto& kv : s->reqs) {  // completed but never waited for
+        if (kv.second->ev) (void)hipEventDestroy(kv.second->ev);
+        delete kv.second;
+    }
     for (hipEvent_t e : s->event_pool) (void)hipEventDestroy(e);
     for (auto& kv : s->plans) fftz::plan_destroy(kv.second);
     s->magic = 0;

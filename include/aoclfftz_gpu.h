@@ -51,7 +51,8 @@ aoclfftz_error_type aoclfftz_gpu_set_graph(FFTZ_VOID *handle, FFTZ_INT32 enable)
 aoclfftz_error_type aoclfftz_gpu_set_compute_mode(FFTZ_VOID *handle, FFTZ_INT32 mode);
 
 /* Autotuner.  Times the handle's candidate tile shapes on the given device
- * buffers (reps executes each; in / out are overwritten) and keeps the
+ * buffers (reps executes each; in / out are overwritten; an in-place handle
+ * takes in == out) and keeps the
  * fastest: row-tile width and column-tile lines for an ordinary plan,
  * workgroups per CU of each tile class for an uneven batch.  The choice is
  * also stored in a process-wide cache keyed by the problem, which later
@@ -135,7 +136,9 @@ FFTZ_INT64 aoclfftz_gpu_batch_plan_json(const aoclfftz_gpu_batch_item *items, FF
  * next to the requests already running; admission is first come, first
  * served, and requests that arrive while others run are admitted as slots
  * and memory free up.  A progress thread owned by the server does the
- * admitting and completion tracking.  No counterpart in the reference (a
+ * admitting and completion tracking; it sleeps while the server is idle and
+ * polls (yielding the core) while requests are in flight.  Plans stay
+ * resident until the server is destroyed (no eviction).  No counterpart in the reference (a
  * CPU library executes one call at a time on the caller's thread). */
 
 /* NULL on bad arguments or when no device is visible. */
