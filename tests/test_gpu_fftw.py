@@ -136,7 +136,8 @@ def test_guru64_dims():
         assert O.rel_l2(y[b * 80:(b + 1) * 80], ref.reshape(-1)) <= tol(80, "d")
 
 
-@pytest.mark.parametrize("shape", [(64,), (16, 20), (8, 6, 12)])
+# odd and prime innermost lengths run the full-length real path
+@pytest.mark.parametrize("shape", [(64,), (16, 20), (8, 6, 12), (63,), (10, 101), (9, 7, 5)])
 def test_r2c_c2r_defaults(shape):
     lib = _lib()
     rank = len(shape)
