@@ -127,3 +127,46 @@ def test_serving_rejects_oversize_and_invalid(torch_gpu):
     assert srv.wait(rid)[0] == 0
     assert srv.test(rid) == -1  # forgotten after wait
     srv.destroy()
+
+
+def test_serving_concurrent_submitters(torch_gpu):
+    """Four host threads submit and wait concurrently (the C calls release the
+    GIL); every result equals the same problem run alone."""
+    import threading
+    torch = torch_gpu
+    n, b = 4096, 4
+    g = torch.Generator(device="cuda")
+    g.manual_seed(12)
+    xs = [torch.complex(torch.rand(n * b, device="cuda", generator=g), torch.rand(n * b, device="cuda", generator=g))
+          for _ in range(32)]
+    ref = []
+    for x in xs:
+        y = torch.empty_like(x)
+        p = fftz.Plan([(n, 1, 1)], [(b, n, n)], "f", 0, inp=x, out=y)
+        assert p.execute() == 0
+        torch.cuda.synchronize()
+        ref.append(y)
+        p.destroy()
+    outs = [torch.empty_like(x) for x in xs]
+    srv = fftz.Server(256 << 20, max_inflight=6)
+    errors = []
+
+    def worker(k):
+        try:
+            for i in range(k, len(xs), 4):
+                rid = srv.submit([(n, 1, 1)], [(b, n, n)], "f", 0, inp=xs[i], out=outs[i])
+                assert rid >= 0
+                assert srv.wait(rid)[0] == 0
+        except Exception as e:  # surfaced below
+            errors.append(e)
+
+    ts = [threading.Thread(target=worker, args=(k,)) for k in range(4)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    assert not errors, errors
+    assert srv.stats()["completed"] == len(xs)
+    for a, c in zip(ref, outs):
+        assert torch.equal(a.view(torch.float32), c.view(torch.float32))
+    srv.destroy()
