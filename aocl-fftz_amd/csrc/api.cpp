@@ -537,6 +537,21 @@ aoclfftz_error_type aoclfftz_gpu_autotune(FFTZ_VOID* handle, FFTZ_VOID* in, FFTZ
 FFTZ_INT32 aoclfftz_gpu_tune_export(const FFTZ_CHAR* path) { return fftz::tune_export(path); }
 FFTZ_INT32 aoclfftz_gpu_tune_import(const FFTZ_CHAR* path) { return fftz::tune_import(path); }
 
+FFTZ_CHAR* aoclfftz_gpu_tune_export_string(FFTZ_VOID)
+{
+    const std::string text = fftz::tune_text(nullptr);
+    char* s = (char*)malloc(text.size() + 1);
+    if (s) memcpy(s, text.c_str(), text.size() + 1);
+    return s;
+}
+
+FFTZ_INT32 aoclfftz_gpu_tune_import_string(const FFTZ_CHAR* text)
+{
+    return text ? fftz::tune_parse(text) : -1;
+}
+
+FFTZ_VOID aoclfftz_gpu_tune_clear(FFTZ_VOID) { fftz::tune_clear(); }
+
 aoclfftz_error_type aoclfftz_gpu_set_compute_mode(FFTZ_VOID* handle, FFTZ_INT32 mode)
 {
     Handle* h = as_handle(handle);

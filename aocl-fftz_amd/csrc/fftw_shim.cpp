@@ -161,6 +161,44 @@ void* unsupported_r2r()
     return nullptr;
 }
 
+// Wisdom is the library's tuned-tile cache as text (one problem per line);
+// the reference's wisdom calls are stubs (src/c/fftw_misc_wrapper.c:300-367):
+// exports succeed with nothing written, imports fail.  Here exports write the
+// cache and imports return 1 when the text parses (0 otherwise, importing
+// nothing).  Both precisions share the cache; its keys carry the precision.
+void wisdom_to_file(FILE* f)
+{
+    char* s = aoclfftz_gpu_tune_export_string();
+    if (s && f) fputs(s, f);
+    free(s);
+}
+
+void wisdom_to_callback(void (*w)(char, void*), void* data)
+{
+    char* s = aoclfftz_gpu_tune_export_string();
+    if (s && w)
+        for (const char* c = s; *c; c++) w(*c, data);
+    free(s);
+}
+
+int wisdom_from_file(FILE* f)
+{
+    if (!f) return 0;
+    std::string text;
+    char buf[4096];
+    size_t got;
+    while ((got = fread(buf, 1, sizeof(buf), f)) > 0) text.append(buf, got);
+    return aoclfftz_gpu_tune_import_string(text.c_str()) >= 0;
+}
+
+int wisdom_from_callback(int (*r)(void*), void* data)
+{
+    if (!r) return 0;
+    std::string text;
+    for (int c = r(data); c != EOF && c != 0; c = r(data)) text.push_back((char)c);
+    return aoclfftz_gpu_tune_import_string(text.c_str()) >= 0;
+}
+
 void* host_alloc(size_t bytes)
 {
     void* p = nullptr;
@@ -332,16 +370,16 @@ char* sprint(void* p)
     void X##fprint_plan(const X##plan p, FILE* f) { print_to(p, f); }                                      \
     void X##print_plan(const X##plan p) { print_to(p, stdout); }                                           \
     char* X##sprint_plan(const X##plan p) { return sprint(p); }                                            \
-    void X##forget_wisdom(void) {}                                                                         \
-    int X##export_wisdom_to_filename(const char*) { return 0; }                                            \
-    void X##export_wisdom_to_file(FILE*) {}                                                                \
-    char* X##export_wisdom_to_string(void) { return nullptr; }                                             \
-    void X##export_wisdom(X##write_char_func, void*) {}                                                    \
+    void X##forget_wisdom(void) { aoclfftz_gpu_tune_clear(); }                                             \
+    int X##export_wisdom_to_filename(const char* f) { return aoclfftz_gpu_tune_export(f) >= 0; }           \
+    void X##export_wisdom_to_file(FILE* f) { wisdom_to_file(f); }                                          \
+    char* X##export_wisdom_to_string(void) { return aoclfftz_gpu_tune_export_string(); }                   \
+    void X##export_wisdom(X##write_char_func w, void* data) { wisdom_to_callback(w, data); }               \
     int X##import_system_wisdom(void) { return 0; }                                                        \
-    int X##import_wisdom_from_filename(const char*) { return 0; }                                          \
-    int X##import_wisdom_from_file(FILE*) { return 0; }                                                    \
-    int X##import_wisdom_from_string(const char*) { return 0; }                                            \
-    int X##import_wisdom(X##read_char_func, void*) { return 0; }                                           \
+    int X##import_wisdom_from_filename(const char* f) { return aoclfftz_gpu_tune_import(f) >= 0; }         \
+    int X##import_wisdom_from_file(FILE* f) { return wisdom_from_file(f); }                                \
+    int X##import_wisdom_from_string(const char* s) { return aoclfftz_gpu_tune_import_string(s) >= 0; }    \
+    int X##import_wisdom(X##read_char_func r, void* data) { return wisdom_from_callback(r, data); }        \
     const char X##version[] = "fftw-3.3.10-aoclfftz-mi355x";
 
 extern "C" {

@@ -1266,37 +1266,65 @@ void tune_store(const std::string& key, const TuneParams& t)
     g_tune[key] = t;
 }
 
+std::string tune_text(int* entries)
+{
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    std::ostringstream os;
+    os << "# fftz tuned tile shapes: key row_t col_c wg_small wg_medium\n";
+    for (const auto& kv : g_tune)
+        os << kv.first << ' ' << kv.second.row_t << ' ' << kv.second.col_c << ' ' << kv.second.wg_small << ' '
+           << kv.second.wg_medium << '\n';
+    if (entries) *entries = (int)g_tune.size();
+    return os.str();
+}
+
+int tune_parse(const std::string& text)
+{
+    // All or nothing: a malformed line leaves the cache untouched.
+    std::vector<std::pair<std::string, TuneParams>> got;
+    std::istringstream is(text);
+    std::string line;
+    while (std::getline(is, line)) {
+        const size_t b = line.find_first_not_of(" \t\r");
+        if (b == std::string::npos || line[b] == '#') continue;
+        std::istringstream ls(line);
+        std::string key, extra;
+        TuneParams t;
+        if (!(ls >> key >> t.row_t >> t.col_c >> t.wg_small >> t.wg_medium) || (ls >> extra)) return -1;
+        for (int v : {t.row_t, t.col_c, t.wg_small, t.wg_medium})
+            if (v < 0 || v > 4096) return -1;
+        got.emplace_back(key, t);
+    }
+    for (const auto& kv : got) tune_store(kv.first, kv.second);
+    return (int)got.size();
+}
+
+void tune_clear()
+{
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    g_tune.clear();
+}
+
 int tune_export(const char* path)
 {
     FILE* f = path ? fopen(path, "w") : nullptr;
     if (!f) return -1;
-    std::lock_guard<std::mutex> g(g_tune_mu);
-    fprintf(f, "# fftz tuned tile shapes: key row_t col_c wg_small wg_medium\n");
-    for (const auto& kv : g_tune)
-        fprintf(f, "%s %d %d %d %d\n", kv.first.c_str(), kv.second.row_t, kv.second.col_c, kv.second.wg_small,
-                kv.second.wg_medium);
-    const int n = (int)g_tune.size();
-    fclose(f);
-    return n;
+    int n = 0;
+    const std::string text = tune_text(&n);
+    const bool ok = fwrite(text.data(), 1, text.size(), f) == text.size();
+    return (fclose(f) == 0 && ok) ? n : -1;
 }
 
 int tune_import(const char* path)
 {
     FILE* f = path ? fopen(path, "r") : nullptr;
     if (!f) return -1;
-    char line[4096];
-    int n = 0;
-    while (fgets(line, sizeof(line), f)) {
-        if (line[0] == '#') continue;
-        char key[4096];
-        TuneParams t;
-        if (sscanf(line, "%4095s %d %d %d %d", key, &t.row_t, &t.col_c, &t.wg_small, &t.wg_medium) == 5) {
-            tune_store(key, t);
-            n++;
-        }
-    }
+    std::string text;
+    char buf[4096];
+    size_t got;
+    while ((got = fread(buf, 1, sizeof(buf), f)) > 0) text.append(buf, got);
     fclose(f);
-    return n;
+    return tune_parse(text);
 }
 
 Plan* plan_create(const ProblemSpec& spec_in, std::string& err, bool dry_run)

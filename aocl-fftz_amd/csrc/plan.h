@@ -148,7 +148,10 @@ std::string batch_key(const std::vector<BatchItem>& items, int prec_bytes, int d
 bool tune_lookup(const std::string& key, TuneParams& out);
 void tune_store(const std::string& key, const TuneParams& t);
 int tune_export(const char* path);  // entries written, -1 on error
-int tune_import(const char* path);  // entries read, -1 on error
+int tune_import(const char* path);  // entries read, -1 on error or a malformed line
+std::string tune_text(int* entries);           // the cache as text, one problem per line
+int tune_parse(const std::string& text);       // entries stored, -1 (nothing stored) if malformed
+void tune_clear();
 
 // Kernel time of `reps` executes of p on (in, out) (device buffers) on a slot
 // stream, in ms; negative on a HIP error.

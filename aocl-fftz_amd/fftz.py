@@ -67,7 +67,8 @@ EXPORTS = (
     "aoclfftz_execute", "aoclfftz_execute_io", "aoclfftz_destroy", "aoclfftz_version",
     "aoclfftz_gpu_set_stream", "aoclfftz_gpu_synchronize", "aoclfftz_gpu_set_graph",
     "aoclfftz_gpu_set_compute_mode", "aoclfftz_gpu_autotune", "aoclfftz_gpu_tune_export",
-    "aoclfftz_gpu_tune_import",
+    "aoclfftz_gpu_tune_import", "aoclfftz_gpu_tune_export_string", "aoclfftz_gpu_tune_import_string",
+    "aoclfftz_gpu_tune_clear",
     "aoclfftz_gpu_server_create", "aoclfftz_gpu_server_submit", "aoclfftz_gpu_server_test",
     "aoclfftz_gpu_server_wait", "aoclfftz_gpu_server_stats", "aoclfftz_gpu_server_reset_stats",
     "aoclfftz_gpu_server_destroy",
@@ -120,6 +121,12 @@ def load():
     lib.aoclfftz_gpu_tune_export.restype = ctypes.c_int32
     lib.aoclfftz_gpu_tune_import.argtypes = [ctypes.c_char_p]
     lib.aoclfftz_gpu_tune_import.restype = ctypes.c_int32
+    lib.aoclfftz_gpu_tune_export_string.argtypes = []
+    lib.aoclfftz_gpu_tune_export_string.restype = vp  # malloc'd; released with libc free
+    lib.aoclfftz_gpu_tune_import_string.argtypes = [ctypes.c_char_p]
+    lib.aoclfftz_gpu_tune_import_string.restype = ctypes.c_int32
+    lib.aoclfftz_gpu_tune_clear.argtypes = []
+    lib.aoclfftz_gpu_tune_clear.restype = None
     lib.aoclfftz_gpu_server_create.argtypes = [ctypes.c_int64, ctypes.c_int32]
     lib.aoclfftz_gpu_server_create.restype = vp
     lib.aoclfftz_gpu_server_submit.argtypes = [vp, vp, ctypes.c_int32]
@@ -391,6 +398,26 @@ def tune_export(path: str) -> int:
 
 def tune_import(path: str) -> int:
     return load().aoclfftz_gpu_tune_import(path.encode())
+
+
+def tune_text() -> str:
+    """The tuned cache as text, one problem per line."""
+    p = load().aoclfftz_gpu_tune_export_string()
+    if not p:
+        raise MemoryError("tune cache export")
+    try:
+        return ctypes.string_at(p).decode()
+    finally:
+        ctypes.CDLL(None).free(ctypes.c_void_p(p))
+
+
+def tune_parse(text: str) -> int:
+    """Import cache entries from text: entries stored, -1 (nothing stored) if malformed."""
+    return load().aoclfftz_gpu_tune_import_string(text.encode())
+
+
+def tune_clear() -> None:
+    load().aoclfftz_gpu_tune_clear()
 
 
 def version() -> str:

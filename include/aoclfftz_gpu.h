@@ -60,12 +60,25 @@ aoclfftz_error_type aoclfftz_gpu_set_compute_mode(FFTZ_VOID *handle, FFTZ_INT32 
  * the reference (its plan choice is a cost model, selector/selector.c). */
 aoclfftz_error_type aoclfftz_gpu_autotune(FFTZ_VOID *handle, FFTZ_VOID *in, FFTZ_VOID *out, FFTZ_INT32 reps);
 
-/* Write / read the tuned cache as text (one problem per line).  Return the
- * number of entries, or -1 when the file cannot be opened.  The counterpart
- * of FFTW wisdom files for this library (the reference's wisdom calls are
- * stubs). */
+/* Write / read the tuned cache as text (one problem per line:
+ * "key row_t col_c wg_small wg_medium"; '#' starts a comment line).  Return
+ * the number of entries, or -1 when the file cannot be opened or a line is
+ * malformed (then nothing is imported).  The counterpart of FFTW wisdom files
+ * for this library; the reference's wisdom calls are stubs
+ * (wrappers/fftw/src/c/fftw_misc_wrapper.c:300-367), this library's FFTW
+ * wrapper routes them here. */
 FFTZ_INT32 aoclfftz_gpu_tune_export(const FFTZ_CHAR *path);
 FFTZ_INT32 aoclfftz_gpu_tune_import(const FFTZ_CHAR *path);
+
+/* The same text in memory.  _export_string returns a malloc'd,
+ * NUL-terminated string (release with free()), NULL only when out of memory;
+ * _import_string returns entries stored or -1 as above; _clear empties the
+ * cache (later setups use the planner's defaults).  No counterparts in the
+ * reference; they back fftw_export_wisdom_to_string / import_wisdom_from_string
+ * / forget_wisdom and the callback forms. */
+FFTZ_CHAR *aoclfftz_gpu_tune_export_string(FFTZ_VOID);
+FFTZ_INT32 aoclfftz_gpu_tune_import_string(const FFTZ_CHAR *text);
+FFTZ_VOID aoclfftz_gpu_tune_clear(FFTZ_VOID);
 
 /* Human-readable plan summary (pass list with lengths, radices, tiles).  The
  * string lives as long as the handle. */
