@@ -274,3 +274,53 @@ def test_autotune_keeps_results(torch_gpu, tmp_path):
         p.destroy()
     path = str(tmp_path / "tune.txt")
     assert fftz.tune_export(path) >= 2
+
+
+def test_wisdom_round_trip_restores_tuned_plan(torch_gpu):
+    """The tuned cache survives export -> forget -> import as FFTW wisdom text:
+    a plan made after the import has the tuned tile shape and gives the tuned
+    plan's result bit for bit.  A tile shape imported by hand (one line per
+    64-thread tile) changes only the grouping of lines, so the result is the
+    same bits as the default shape's."""
+    import ctypes
+    torch = torch_gpu
+    lib = fftz.load()
+    lib.fftw_export_wisdom_to_string.restype = ctypes.c_void_p
+    lib.fftw_import_wisdom_from_string.argtypes = [ctypes.c_char_p]
+    n, b = 1024, 512
+    key = f"p4:0:0:0:0|{n},1,1;|{b},{n},{n};"
+    x = rand_c((b, n), 19, "f")
+    dx = to_dev(torch, x)
+
+    def run():
+        dy = torch.empty_like(dx)
+        p = fftz.Plan([(n, 1, 1)], [(b, n, n)], "f", 0, inp=dx, out=dy)
+        assert p.execute() == 0
+        torch.cuda.synchronize()
+        info = p.info
+        p.destroy()
+        return info, dy
+
+    lib.fftw_forget_wisdom()
+    info_default, y_default = run()
+    scratch_in, scratch_out = dx.clone(), torch.empty_like(dx)
+    p = fftz.Plan([(n, 1, 1)], [(b, n, n)], "f", 0, inp=dx, out=torch.empty_like(dx))
+    assert p.autotune(scratch_in, scratch_out, reps=3) == 0
+    p.destroy()
+    info_tuned, y_tuned = run()
+    s = lib.fftw_export_wisdom_to_string()
+    text = ctypes.string_at(s)
+    ctypes.CDLL(None).free(ctypes.c_void_p(s))
+    assert key.encode() in text
+    lib.fftw_forget_wisdom()
+    assert run()[0] == info_default
+    assert lib.fftw_import_wisdom_from_string(text) == 1
+    info_again, y_again = run()
+    assert info_again == info_tuned
+    assert torch.equal(y_again.view(torch.float32), y_tuned.view(torch.float32))
+    assert lib.fftw_import_wisdom_from_string(f"{key} 64 0 0 0\n".encode()) == 1
+    info_64, y_64 = run()
+    assert "C=1 " in info_64, info_64
+    assert torch.equal(y_64.view(torch.float32), y_default.view(torch.float32))
+    assert O.rel_l2(y_64.cpu().numpy(), O.fft(x.astype(np.complex128)).reshape(-1)) <= tol(n, "f")
+    lib.fftw_forget_wisdom()
