@@ -89,3 +89,34 @@ def test_fftw_wisdom_calls_use_the_cache(tmp_path):
     assert lines_per_tile() == 4
     assert lib.fftw_import_system_wisdom() == 0
     lib.fftw_forget_wisdom()
+
+
+def test_fftw_wisdom_file_forms(tmp_path):
+    """fftw_export_wisdom_to_file / fftw_import_wisdom_from_file on C FILE*
+    streams opened by the caller."""
+    import ctypes
+    lib = fftz.load()
+    libc = ctypes.CDLL(None)
+    libc.fopen.restype = ctypes.c_void_p
+    libc.fopen.argtypes = [ctypes.c_char_p, ctypes.c_char_p]
+    libc.fclose.argtypes = [ctypes.c_void_p]
+    lib.fftwf_export_wisdom_to_file.argtypes = [ctypes.c_void_p]
+    lib.fftw_import_wisdom_from_file.argtypes = [ctypes.c_void_p]
+    lib.fftw_import_wisdom_from_string.argtypes = [ctypes.c_char_p]
+    lib.fftw_forget_wisdom()
+    assert lib.fftw_import_wisdom_from_string(f"{KEY} 64 0 0 0\n".encode()) == 1
+    path = str(tmp_path / "w.txt").encode()
+    fp = libc.fopen(path, b"w")
+    assert fp
+    lib.fftwf_export_wisdom_to_file(fp)
+    assert libc.fclose(fp) == 0
+    lib.fftw_forget_wisdom()
+    assert lines_per_tile() == 4
+    fp = libc.fopen(path, b"r")
+    assert fp
+    assert lib.fftw_import_wisdom_from_file(fp) == 1
+    assert libc.fclose(fp) == 0
+    assert lines_per_tile() == 1
+    assert lib.fftw_import_wisdom_from_file(None) == 0
+    lib.fftw_forget_wisdom()
+    assert lines_per_tile() == 4
