@@ -540,8 +540,10 @@ static bool build_pass(Plan* plan, TableBuilder& tb, const PassReq& q, std::stri
 
     // expand host-looped axes
     std::vector<int64_t> ioffs{0}, ooffs{0};
-    const int64_t in_scale = ((q.flags & kRealIn) ? prec : esize);
-    const int64_t out_scale = ((q.flags & kRealOut) ? prec : esize);
+    // real-side strides count real elements (packed R2C / C2R and the
+    // full-length promoted / real-part forms alike)
+    const int64_t in_scale = ((q.flags & (kRealIn | kRealPromote)) ? prec : esize);
+    const int64_t out_scale = ((q.flags & (kRealOut | kRealPart)) ? prec : esize);
     for (auto& h : host_axes) {
         std::vector<int64_t> ni, no;
         for (size_t k = 0; k < ioffs.size(); k++)

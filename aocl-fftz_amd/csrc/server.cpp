@@ -129,21 +129,25 @@ void admit_and_reap(Server& s)
         Req* r = s.queue.front();
         if (!s.running.empty() && s.in_use + r->footprint > s.budget) break;  // FIFO: wait for memory
         s.queue.pop_front();
+        // a request that cannot start completes as failed, with its
+        // completion time set so its reported latency stays meaningful
+        auto fail = [&](Req* q) {
+            q->t_done = now_s();
+            q->state = kFailed;
+            s.failed++;
+            s.cv_done.notify_all();
+        };
         hipError_t e = hipSuccess;
         r->slot = plan_acquire_slot(r->plan, &e);
         if (!r->slot) {
-            r->state = kFailed;
-            s.failed++;
-            s.cv_done.notify_all();
+            fail(r);
             continue;
         }
         if (s.event_pool.empty()) {
             hipEvent_t ev;
             if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
                 plan_release_slot(r->plan, r->slot);
-                r->state = kFailed;
-                s.failed++;
-                s.cv_done.notify_all();
+                fail(r);
                 continue;
             }
             s.event_pool.push_back(ev);
@@ -154,10 +158,11 @@ void admit_and_reap(Server& s)
         if (e == hipSuccess) e = hipEventRecord(r->ev, r->slot->stream);
         if (e != hipSuccess) {
             (void)hipGetLastError();
+            (void)hipStreamSynchronize(r->slot->stream);
+            s.event_pool.push_back(r->ev);
+            r->ev = nullptr;
             plan_release_slot(r->plan, r->slot);
-            r->state = kFailed;
-            s.failed++;
-            s.cv_done.notify_all();
+            fail(r);
             continue;
         }
         r->state = kRunning;
@@ -263,7 +268,10 @@ FFTZ_INT64 aoclfftz_gpu_server_submit(FFTZ_VOID* server, FFTZ_VOID* problem, FFT
     r->plan = plan;
     r->in = in;
     r->out = out;
-    const size_t ext = (size_t)plan->in_extent_bytes + (spec.inplace ? 0 : (size_t)plan->out_extent_bytes);
+    // in place, the one buffer spans the larger of the two extents (the
+    // complex side of an in-place R2C / C2R is the larger)
+    const size_t ext = spec.inplace ? (size_t)std::max(plan->in_extent_bytes, plan->out_extent_bytes)
+                                    : (size_t)plan->in_extent_bytes + (size_t)plan->out_extent_bytes;
     r->footprint = ext + plan->work_bytes;
     r->transforms = 1;
     for (const auto& v : spec.vecs) r->transforms *= v.n;

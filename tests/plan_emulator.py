@@ -13,6 +13,7 @@ import numpy as np
 
 K_CONJ_IN, K_CONJ_OUT, K_TW4 = 1, 2, 4
 K_SCALE, K_R2C_POST, K_C2R_PRE, K_REAL_IN, K_REAL_OUT = 32, 128, 256, 512, 1024
+K_REAL_PROMOTE, K_HERM_IN, K_REAL_PART = 2048, 4096, 8192
 
 
 class OOB(AssertionError):
@@ -45,8 +46,11 @@ def run(plan, bufs):
         flags = ln["flags"]
         nl = ln["nlines"]
         ioff, ooff, l0 = _line_offsets(ln, nl)
-        in_es = src.itemsize
-        out_es = dst.itemsize
+        # launch offsets are bytes in the plan's precision: complex elements
+        # are 2 * prec bytes, real elements prec bytes
+        prec = plan["prec"]
+        in_es = prec * (2 if np.iscomplexobj(src) else 1)
+        out_es = prec * (2 if np.iscomplexobj(dst) else 1)
         assert ln["in_off"] % in_es == 0 and ln["out_off"] % out_es == 0
         ioff = ioff + ln["in_off"] // in_es
         ooff = ooff + ln["out_off"] // out_es
@@ -57,6 +61,24 @@ def run(plan, bufs):
             _check(ire, src.size, "real load")
             _check(iim, src.size, "real load")
             x = src[ire] + 1j * src[iim]
+        elif flags & K_REAL_PROMOTE:
+            # full-length R2C: real element e loaded as (x[e], 0)
+            nv = ln["nvalid_in"]
+            e = np.arange(nv, dtype=np.int64)
+            idx = ioff[:, None] + e[None, :] * ln["ies"]
+            _check(idx, src.size, "real promote load")
+            x = np.zeros((nl, max(L, nv)), np.complex128)
+            x[:, :nv] = src[idx]
+        elif flags & K_HERM_IN:
+            # full-length C2R: bins 0..n/2 stored, bin e > n/2 is conj(bin n-e)
+            n = ln["nvalid_in"]
+            e = np.arange(n, dtype=np.int64)
+            m = np.where(2 * e <= n, e, n - e)
+            idx = ioff[:, None] + m[None, :] * ln["ies"]
+            _check(idx, src.size, "hermitian load")
+            x = np.zeros((nl, max(L, n)), np.complex128)
+            v = src[idx]
+            x[:, :n] = np.where((2 * e > n)[None, :], np.conj(v), v)
         else:
             nv = ln["nvalid_in"]
             e = np.arange(nv, dtype=np.int64)
@@ -71,10 +93,11 @@ def run(plan, bufs):
             y = np.fft.fft(x[:, :n], axis=1)
             if flags & K_CONJ_OUT:
                 y = np.conj(y)
-            e = np.arange(n, dtype=np.int64)
+            nvo = ln["nvalid_out"]
+            e = np.arange(nvo, dtype=np.int64)
             idx = ooff[:, None] + e[None, :] * ln["oes"]
             _check(idx, dst.size, "bluestein store")
-            dst[idx] = y
+            dst[idx] = y[:, :nvo].real if flags & K_REAL_PART else y[:, :nvo]
             continue
         if flags & K_C2R_PRE:
             X = x[:, :L + 1]
@@ -105,7 +128,11 @@ def run(plan, bufs):
             y = y * np.exp(-2j * np.pi * p / N)
         if flags & K_CONJ_OUT:
             y = np.conj(y)
-        if flags & K_REAL_OUT:
+        if flags & K_REAL_PART:
+            idx = ooff[:, None] + e[None, :] * ln["oes"]
+            _check(idx, dst.size, "real-part store")
+            dst[idx] = y.real
+        elif flags & K_REAL_OUT:
             ire = ooff[:, None] + (2 * e)[None, :] * ln["oes"]
             iim = ooff[:, None] + (2 * e + 1)[None, :] * ln["oes"]
             _check(ire, dst.size, "real store")

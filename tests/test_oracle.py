@@ -147,6 +147,77 @@ def test_nd_strided():
         assert O.rel_l2(out[i::2 * b][:n], np.fft.fft(seq)) < 1e-14
 
 
+# --- error metrics: values fixed by hand ------------------------------------
+# orc_rel_l2 is the metric behind every parity assertion, so it is pinned to
+# values worked out on paper, on inputs chosen so that the plausible mistakes
+# give a different number: a missing square root (5/9 instead of sqrt(5/9)),
+# dividing by ||a|| instead of ||b|| (sqrt(5/6)), or a max-norm ratio (2/sqrt(8)).
+
+def test_rel_l2_hand_values():
+    a = np.array([1 + 1j, 2 + 0j])
+    b = np.array([1 + 0j, 2 + 2j])
+    # a - b = (i, -2i): |a-b|^2 = 1 + 4 = 5; |b|^2 = 1 + 8 = 9
+    assert O.rel_l2(a, b) == pytest.approx(np.sqrt(5.0 / 9.0), rel=1e-15)
+    for wrong in (5.0 / 9.0, np.sqrt(5.0 / 6.0), 2.0 / np.sqrt(8.0)):
+        assert abs(O.rel_l2(a, b) - wrong) > 1e-3
+    # real arrays: (3, 0) vs (0, 4) -> sqrt(9 + 16) / 4 = 1.25
+    assert O.rel_l2(np.array([3.0, 0.0]), np.array([0.0, 4.0])) == pytest.approx(1.25, rel=1e-15)
+    # the metric is not symmetric: swapping the operands divides by ||a|| = 3
+    assert O.rel_l2(np.array([0.0, 4.0]), np.array([3.0, 0.0])) == pytest.approx(5.0 / 3.0, rel=1e-15)
+    # zero reference: the absolute L2 norm of the difference
+    assert O.rel_l2(np.array([3.0, 4.0]), np.zeros(2)) == pytest.approx(5.0, rel=1e-15)
+    assert O.rel_l2(np.zeros(3), np.zeros(3)) == 0.0
+    # equal arrays
+    assert O.rel_l2(b, b) == 0.0
+
+
+def test_rel_l2_sums_every_element():
+    # one differing element among many: sqrt(1 / (n * 4)) with b = 2 everywhere;
+    # a metric that looked at only the first or last element would miss it
+    n = 1001
+    b = np.full(n, 2.0 + 0j)
+    a = b.copy()
+    a[517] += 1.0
+    assert O.rel_l2(a, b) == pytest.approx(np.sqrt(1.0 / (4.0 * n)), rel=1e-14)
+    a = b.copy()
+    a[3] += 1j
+    a[900] -= 2.0
+    assert O.rel_l2(a, b) == pytest.approx(np.sqrt(5.0 / (4.0 * n)), rel=1e-14)
+
+
+def test_rel_l2_resolves_tiny_differences():
+    # an FP64 ulp-level difference must not vanish (long-double accumulation)
+    b = np.ones(4, dtype=np.complex128)
+    a = b.copy()
+    a[0] = 1.0 + 2.0 ** -52
+    # |a - b| = 2^-52 over ||b|| = sqrt(4) = 2 (four complex ones)
+    assert O.rel_l2(a, b) == pytest.approx(2.0 ** -52 / 2.0, rel=1e-12)
+
+
+def test_ref_rel_err_hand_values():
+    # compare.c:88-147: max over scalars of |a_i - b_i|, divided by the largest
+    # min(|a_i|, |b_i|) over scalars.
+    # scalars a = (4, 1), b = (1, 2): errors (3, 1) -> 3; mins (1, 1) -> 1 => 3.
+    # Wrong forms give 1.5 (max |b|), 0.75 (max |a|), sqrt(10/5) (L2).
+    a = np.array([4.0 + 1.0j])
+    b = np.array([1.0 + 2.0j])
+    assert O.ref_rel_err(a, b) == pytest.approx(3.0, rel=1e-15)
+    # scalars a = (1, -2, 5, 0.5), b = (1.5, -2, 3, 0.5): max err 2; mins
+    # (1, 2, 3, 0.5) -> 3 => 2/3
+    a = np.array([1 - 2j, 5 + 0.5j])
+    b = np.array([1.5 - 2j, 3 + 0.5j])
+    assert O.ref_rel_err(a, b) == pytest.approx(2.0 / 3.0, rel=1e-15)
+    # sign of the error does not matter, and the real and imaginary parts are
+    # separate scalars (not complex moduli): (0 + 3i) vs (0 + 0i) with a
+    # companion element of magnitude 4 -> 3 / 4
+    a = np.array([0 + 3j, 4 + 0j])
+    b = np.array([0 + 0j, 4 + 0j])
+    assert O.ref_rel_err(a, b) == pytest.approx(0.75, rel=1e-15)
+    # all-zero magnitudes: the absolute error is returned; all zero: 0
+    assert O.ref_rel_err(np.array([1.0 + 0j]), np.array([0j])) == pytest.approx(1.0)
+    assert O.ref_rel_err(np.zeros(2, complex), np.zeros(2, complex)) == 0.0
+
+
 # --- planted mistakes: each must be caught by the checks above -------------
 
 def test_planted_sign_error_is_caught():

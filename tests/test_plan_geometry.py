@@ -153,3 +153,41 @@ def test_cfg4a_plan_shape():
     plan = fftz.plan_json(dims, [(1, n ** 3, n * n * h)], "f", 0, fft_type=1)
     emu.check_launch_limits(plan)
     assert len(plan["launches"]) == 3
+
+
+def _vec_offsets(vecs, side):
+    """Element offsets of every batch index (side 1: input strides, 2: output)."""
+    offs = [0]
+    for v in vecs:
+        offs = [o + j * v[side] for o in offs for j in range(v[0])]
+    return offs
+
+
+@pytest.mark.parametrize("n", [15, 17, 64])
+@pytest.mark.parametrize("prec", ["f", "d"])
+def test_real_full_length_many_batch_axes(n, prec):
+    """R2C / C2R with four batch axes that cannot merge: more than three line
+    levels, so the outer axes are looped on the host and their offsets are
+    counted in real elements on the real side (odd n: the full-length
+    promoted / real-part forms; n = 17: Bluestein; n = 64: real-packed)."""
+    h = n // 2 + 1
+    r, c = n + 5, h + 2  # gaps between batches on both sides
+    vecs_r2c = [(2, r, c), (2, 10 * r, 10 * c), (2, 100 * r, 100 * c), (2, 1000 * r, 1000 * c)]
+    plan = fftz.plan_json([(n, 1, 1)], vecs_r2c, prec, 0, fft_type=1)
+    emu.check_launch_limits(plan)
+    assert len(plan["launches"]) >= 2  # host-looped outer axis
+    rng = np.random.default_rng(n)
+    xr = rng.standard_normal(plan["in_extent_bytes"] // plan["prec"])
+    out = np.zeros(plan["out_extent_bytes"] // (2 * plan["prec"]), np.complex128)
+    emu.run(plan, {0: xr, 1: out, 2: np.zeros(1, np.complex128)})
+    for oi, oo in zip(_vec_offsets(vecs_r2c, 1), _vec_offsets(vecs_r2c, 2)):
+        assert O.rel_l2(out[oo:oo + h], np.fft.rfft(xr[oi:oi + n])) < 1e-12
+    # C2R: the same tensor read from the complex side
+    vecs_c2r = [(v[0], v[2], v[1]) for v in vecs_r2c]
+    plan_b = fftz.plan_json([(n, 1, 1)], vecs_c2r, prec, 1, fft_type=1)
+    emu.check_launch_limits(plan_b)
+    back = np.zeros(plan_b["out_extent_bytes"] // plan_b["prec"])
+    assert back.size == xr.size
+    emu.run(plan_b, {0: out.copy(), 1: back, 2: np.zeros(max(1, plan_b["work_bytes"] // 16), np.complex128)})
+    for oi in _vec_offsets(vecs_r2c, 1):
+        assert np.max(np.abs(back[oi:oi + n] / n - xr[oi:oi + n])) < 1e-10
