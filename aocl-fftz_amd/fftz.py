@@ -276,6 +276,10 @@ class Plan:
         vals = list(buf[:min(n, cap)])
         return [vals[i:i + per] for i in range(0, len(vals), per)]
 
+    def launches(self) -> int:
+        """Kernel launches per execute."""
+        return int(load().aoclfftz_gpu_plan_launches(self.handle))
+
     @property
     def info(self) -> str:
         return load().aoclfftz_gpu_plan_info(self.handle).decode()

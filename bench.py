@@ -22,11 +22,18 @@ Also measured in the same run and reported in the same JSON line:
   * serving: open-loop Poisson arrivals (2k, 10k, 50k requests/s) of a mix
     of 1-D C2C, 2-D R2C and 3-D C2C requests through the serving runtime,
     p50 / p99 latency and transforms/s;
-  * roofline of the dominant kernel: algorithmic bytes per launch over its
-    average duration measured with HIP events on its own stream;
-  * the reference's CPU path (AOCL-FFTZ built from its sources, OpenMP) timed
-    on this host on a bounded sample of the same workload, with the relative
-    L2 difference between the GPU and the reference on the same input.
+  * roofline of the headline: algorithmic bytes of one transform (every
+    element read once and written once, 2 x 16 B x 2^24) over the average
+    duration of one transform (all its launches) measured with HIP events on
+    the launch stream, plus each pass's own bandwidth;
+  * HBM traffic per launch from two rocprofv3 --pmc runs of this file
+    (FETCH_SIZE, then WRITE_SIZE), scaled by the same counters' reading for a
+    plain copy of known size -- child processes started before the GPU is
+    used (--no-traffic skips them);
+  * the reference's CPU path: its own benchmark program (aocl_fftz_bench,
+    built from its sources with OpenMP) on this host at 1 thread and at the
+    host's thread count, and the relative L2 difference between the GPU and
+    the reference library on the same input (--no-cpu skips these).
 
 One GPU only: --gpus above 1 exits with a message.
 """
@@ -48,14 +55,132 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def _traffic(kernel):
-    """HBM bytes per dispatch of `kernel` measured with rocprofv3 FETCH_SIZE /
-    WRITE_SIZE (calibrated on a plain copy), from the committed summary."""
-    try:
-        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
-            return json.load(f).get(kernel)
-    except OSError:
-        return None
+OUR_KERNELS = ("k_fixed_pass", "k_fft_pass", "k_bluestein", "k_batch", "k_mfma_pass", "k_persist")
+
+
+def _is_ours(name):
+    return any(k in name for k in OUR_KERNELS)
+
+
+def traffic_probe(manifest_path):
+    """Run under `rocprofv3 --pmc <counter>` by measure_traffic(): a plain
+    256 MiB device copy (the calibration kernel of known size), then two
+    forward and two backward executes of the headline 2^24 complex128
+    transform, then three executes of the batched N=1024 x 4096 complex64
+    plan.  Writes the dispatch manifest (which of this library's dispatches
+    belong to which workload) as JSON."""
+    import torch
+    import fftz
+    assert torch.cuda.is_available()
+    fftz.load()
+    mib = 256
+    a = torch.ones(mib << 18, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
+    man = {"copy_bytes": mib << 20, "workloads": []}
+    n = 1 << 24
+    x = torch.randn(n, dtype=torch.complex128, device="cuda")
+    fwd = fftz.Plan([(n, 1, 1)], [(1, n, n)], "d", 0, inplace=True, inp=x, out=x)
+    bwd = fftz.Plan([(n, 1, 1)], [(1, n, n)], "d", 1, inplace=True, inp=x, out=x)
+    nl = fwd.launches()
+    for _ in range(2):
+        assert fwd.execute() == 0
+    for _ in range(2):
+        assert bwd.execute() == 0
+    torch.cuda.synchronize()
+    man["workloads"].append({"name": "2p24", "launches": nl, "executes": 4})
+    fwd.destroy()
+    bwd.destroy()
+    del x
+    xs = torch.rand(4096, 1024, dtype=torch.complex64, device="cuda")
+    ys = torch.empty_like(xs)
+    p = fftz.Plan([(1024, 1, 1)], [(4096, 1024, 1024)], "f", 0)
+    for _ in range(3):
+        assert p.execute_io(xs, ys) == 0
+    torch.cuda.synchronize()
+    man["workloads"].append({"name": "n1024", "launches": p.launches(), "executes": 3})
+    p.destroy()
+    with open(manifest_path, "w") as f:
+        json.dump(man, f)
+
+
+def _counter_rows(outdir, counter):
+    """(dispatch id, kernel name, value) for one counter from rocprofv3's
+    counter_collection CSV under outdir, one row per dispatch (instances
+    summed)."""
+    import csv
+    import glob
+    files = glob.glob(os.path.join(outdir, "**", "*counter_collection.csv"), recursive=True)
+    per = {}
+    for fn in files:
+        with open(fn, newline="") as f:
+            for row in csv.DictReader(f):
+                r = {k.lower(): v for k, v in row.items() if k}
+                if r.get("counter_name", "") != counter:
+                    continue
+                key = (int(r.get("process_id", 0) or 0), int(r["dispatch_id"]))
+                name = r.get("kernel_name", "")
+                v = float(r["counter_value"])
+                if key in per:
+                    per[key] = (name, per[key][1] + v)
+                else:
+                    per[key] = (name, v)
+    return [(k[1], v[0], v[1]) for k, v in sorted(per.items())]
+
+
+def measure_traffic(budget_s=90):
+    """HBM bytes per launch of the headline transform and of the batched
+    N=1024 plan, from two rocprofv3 --pmc runs of traffic_probe (FETCH_SIZE,
+    then WRITE_SIZE, each in KiB per dispatch), each scaled by what the same
+    counter reads for the plain copy kernel of known size.  Runs before this
+    process touches the GPU.  Returns (result dict or None, note)."""
+    import shutil
+    import subprocess
+    import tempfile
+    prof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(prof):
+        return None, "rocprofv3 not found"
+    work = tempfile.mkdtemp(prefix="fftz_traffic_")
+    vals = {}
+    man = None
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        outdir = os.path.join(work, counter)
+        mpath = os.path.join(work, counter + ".json")
+        cmd = ["timeout", "-s", "KILL", str(budget_s), prof, "--pmc", counter, "-d", outdir, "-o", "run",
+               "--output-format", "csv", "--", sys.executable, os.path.abspath(__file__),
+               "--traffic-probe", mpath]
+        log(f"[bench] traffic: rocprofv3 --pmc {counter}")
+        t0 = time.perf_counter()
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+        log(f"[bench] traffic: {counter} pass rc={r.returncode} in {time.perf_counter() - t0:.1f} s")
+        if not os.path.exists(mpath):
+            return None, f"{counter} probe failed (rc {r.returncode}): {r.stdout[-400:]}"
+        with open(mpath) as f:
+            man = json.load(f)
+        rows = _counter_rows(outdir, counter)
+        copies = [v for _, name, v in rows if "copyBuffer" in name]
+        ours = [v for _, name, v in rows if _is_ours(name)]
+        if not copies or not ours:
+            return None, f"{counter}: no copy or FFT dispatches in the counter CSV"
+        # the 256 MiB copy is the largest copy dispatch
+        factor = man["copy_bytes"] / (max(copies) * 1024.0)
+        vals[counter] = (factor, [v * 1024.0 * factor for v in ours])
+    res = {"copy_factor": {c: round(vals[c][0], 4) for c in vals}}
+    i = 0
+    for w in man["workloads"]:
+        k, e = w["launches"], w["executes"]
+        per_launch = []
+        for j in range(k):
+            fetch = [vals["FETCH_SIZE"][1][i + x * k + j] for x in range(e)]
+            write = [vals["WRITE_SIZE"][1][i + x * k + j] for x in range(e)]
+            per_launch.append({"read": sum(fetch) / e, "write": sum(write) / e})
+        i += k * e
+        res[w["name"]] = per_launch
+    if i != len(vals["FETCH_SIZE"][1]) or i != len(vals["WRITE_SIZE"][1]):
+        return None, f"dispatch count mismatch: manifest {i}, counters {len(vals['FETCH_SIZE'][1])}"
+    return res, "measured in this run"
 
 
 def launch_bytes(plan_json):
@@ -132,6 +257,10 @@ def bench_roundtrip_2p24(torch, fftz, steps, warmup, n=1 << 24):
     rt_err = (torch.linalg.vector_norm(back - x0) / torch.linalg.vector_norm(x0)).item()
     bpl = launch_bytes(pj)
     kidx, avg, achieved = roofline_from_profile(prof_f + prof_b, bpl)
+    import numpy as np
+    # one transform = all launches of one execute: the sum of its launch
+    # durations (events bracket every launch on the launch stream)
+    per_transform = np.array([sum(e) for e in prof_f + prof_b], dtype=np.float64)
     flops = 2 * 5.0 * n * math.log2(n)
     res = {
         "ms_per_step": ms,
@@ -140,9 +269,12 @@ def bench_roundtrip_2p24(torch, fftz, steps, warmup, n=1 << 24):
         "roundtrip_rel_l2": rt_err,
         "launch_ms_avg": [round(v, 4) for v in avg.tolist()],
         "launch_bytes": bpl,
+        "launch_kernels": [kernel_name(ln, "double") for ln in pj["launches"]],
         "dominant_launch": kidx,
         "dominant_achieved_Bps": achieved,
         "dominant_kernel": kernel_name(pj["launches"][kidx], "double"),
+        "transform_ms_avg": float(per_transform.mean()),
+        "transform_bytes": 2 * 16.0 * n,
         "transform_frac_of_hbm": (2 * 16.0 * n * 2) / (ms * 1e-3) / HBM_PEAK,
         "plan": fwd.info,
     }
@@ -269,8 +401,9 @@ def bench_mixed_radix(torch, fftz, steps, warmup, n=3 << 20):
 
 def bench_r2c_512(torch, fftz, steps, warmup, n=512):
     """3-D R2C float32 512^3 -> complex64 512 x 512 x 257 (real-packed row pass
-    + two column passes); error against the library's float64 transform of
-    the same input."""
+    + two column passes); error against direct DFT sums at sampled bins
+    (float64 torch reductions) and against the library's float64 transform
+    of the same input."""
     g = torch.Generator(device="cuda")
     g.manual_seed(4)
     x = torch.randn((n, n, n), dtype=torch.float32, device="cuda", generator=g)
@@ -302,9 +435,27 @@ def bench_r2c_512(torch, fftz, steps, warmup, n=512):
     torch.cuda.synchronize()
     q.destroy()
     err = (torch.linalg.vector_norm(y.to(torch.complex128) - yd) / torch.linalg.vector_norm(yd)).item()
+    # the definition at sampled bins, X[k2,k1,k0] = sum x[j2,j1,j0] w^(...), as
+    # float64 torch reductions (independent of this library), phases reduced
+    # exactly in integers; error relative to the spectrum's RMS
+    idx = torch.arange(n, device="cuda", dtype=torch.int64)
+
+    def ph(k):
+        return torch.polar(torch.ones(n, device="cuda", dtype=torch.float64),
+                           (-2 * math.pi / n) * ((idx * k) % n).double())
+
+    bins = [(0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1), (256, 256, 256), (3, 100, 200), (129, 7, 511),
+            (13, 400, 100)]
+    rms = (torch.linalg.vector_norm(yd) / math.sqrt(yd.numel())).item()
+    worst = 0.0
+    for k0, k1, k2 in bins:
+        v = (xd.reshape(-1, n).to(torch.complex128) @ ph(k0)).reshape(n, n)
+        ref = ((v @ ph(k1)) @ ph(k2)).item()
+        worst = max(worst, abs(complex(y[k2, k1, k0].item()) - ref) / rms)
     byts = 4.0 * n ** 3 + 8.0 * n * n * h
     res = {"ms_per_execute": round(ms, 4), "gflops": round(2.5 * n ** 3 * math.log2(n ** 3) / (ms * 1e-3) / 1e9, 1),
-           "algorithmic_frac_of_hbm": round(byts / (ms * 1e-3) / HBM_PEAK, 4), "rel_l2_vs_fp64": err, "plan": info}
+           "algorithmic_frac_of_hbm": round(byts / (ms * 1e-3) / HBM_PEAK, 4), "rel_l2_vs_fp64": err,
+           "direct_bins_max_err_over_rms": worst, "plan": info}
     del x, y, xd, yd
     torch.cuda.empty_cache()
     return res
@@ -394,8 +545,9 @@ def sinusoid_lines(torch, n, b, seed=4, tones=8, noise=1e-3, chunk=512):
 def bench_split(torch, fftz, steps, warmup, n=4096, b=16384):
     """Precision workload: batched complex64 C2C, N=4096, batch 16384, lines of
     8 tones + 1e-3 noise; the plain FP32 pass against the matrix-core forms
-    (FP32 operands, bf16 hi+lo, scaled fp16 hi+lo).  Error of each against the
-    library's own complex128 transform of the same (upcast) input."""
+    (FP32 operands, bf16 hi+lo, scaled fp16 hi+lo).  Error of each against
+    numpy's FP64 transform of 256 sampled lines of the same (upcast) input,
+    and over all lines against the library's complex128 transform."""
     x = sinusoid_lines(torch, n, b)
     xd = x.to(torch.complex128)
     yd = torch.empty_like(xd)
@@ -407,6 +559,9 @@ def bench_split(torch, fftz, steps, warmup, n=4096, b=16384):
     pd.destroy()
     del xd
     y = torch.empty_like(x)
+    import numpy as np
+    sample = torch.arange(0, b, b // 256, device="cuda")[:256]
+    ref_np = np.fft.fft(x[sample].cpu().numpy().astype(np.complex128), axis=1)
     flops = 5.0 * n * math.log2(n) * b
     byts = 2 * 8.0 * n * b
     names = {0: "plain_fp32", 1: "mfma_fp32", 2: "mfma_bf16x3", 3: "mfma_fp16x2"}
@@ -431,9 +586,14 @@ def bench_split(torch, fftz, steps, warmup, n=4096, b=16384):
         err = 0.0
         for s in range(0, b, 2048):
             err += torch.linalg.vector_norm(y[s:s + 2048].to(torch.complex128) - yd[s:s + 2048]).item() ** 2
+        # independent FP64 reference on 256 sampled lines: numpy's pocketfft
+        # of the same (upcast) input
+        ys = y[sample].cpu().numpy().astype(np.complex128)
+        es = float(np.linalg.norm(ys - ref_np) / np.linalg.norm(ref_np))
         out[names[mode]] = {"ms": round(ms, 4), "gflops": round(flops / (ms * 1e-3) / 1e9, 1),
                             "frac_of_hbm": round(byts / (ms * 1e-3) / HBM_PEAK, 4),
-                            "rel_l2_vs_fp64": float(math.sqrt(err) / ydn)}
+                            "rel_l2_vs_fp64": float(math.sqrt(err) / ydn),
+                            "rel_l2_vs_numpy_fp64_256_lines": es}
         log(f"[bench] split n4096 {names[mode]}: {ms * 1e3:.1f} us rel-L2 {out[names[mode]]['rel_l2_vs_fp64']:.3e}")
         p.destroy()
     del x, y, yd
@@ -534,46 +694,74 @@ def copy_ceiling(torch, mib=256, reps=20):
     return 2 * mib * (1 << 20) / (ms * 1e-3)
 
 
-def cpu_baseline(res_2p24, res_1024, budget_s):
-    """The reference's OpenMP CPU path, timed on this host (bounded sample)."""
+def _ref_bench_time(args, timeout=240):
+    """Min execution time (seconds) printed by the reference's own benchmark
+    program (oracle/_ref/aocl_fftz_bench: calibrated rounds of >= 100 ms,
+    two warm-ups, min over the timed iterations, test/bench/performance.c:36-125)."""
+    import re
+    import subprocess
+    exe = os.path.join(ROOT, "oracle", "_ref", "aocl_fftz_bench")
+    r = subprocess.run([exe] + args, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=timeout)
+    m = re.search(r"Min Execution time\s*:\s*([0-9.eE+-]+)\s*(ns|us|ms|s)\b", r.stdout)
+    if r.returncode != 0 or not m:
+        raise RuntimeError(f"reference bench {' '.join(args)} failed (rc {r.returncode}): {r.stdout[-300:]}")
+    return float(m.group(1)) * {"ns": 1e-9, "us": 1e-6, "ms": 1e-3, "s": 1.0}[m.group(2)]
+
+
+def cpu_baseline():
+    """The reference's OpenMP CPU path on this host's cores, through its own
+    benchmark program built from its sources (BASELINE.md section 2 steps
+    2-3): the headline 2^24 complex128 in-place transform and the batched
+    N=1024 x 4096 complex64 transform, each at 1 thread and at the host's
+    thread count(s).  Runs before this process touches the GPU."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "aocl_fftz_bench")
+    if not os.path.exists(exe):
+        return None, {"cpu_baseline_note": "oracle/_ref/aocl_fftz_bench not built: CPU baseline unavailable"}
+    avail = len(os.sched_getaffinity(0))
+    omp = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    threads = sorted({1, avail} | ({omp} if 0 < omp <= avail else set()))
+    cfgs = {
+        "2p24_fp64_inplace": (["16777216", "-b", "p", "-p", "d", "-o", "3", "-r", "i"], 5.0 * (1 << 24) * 24),
+        "n1024_b4096_fp32": (["4096v1024", "-b", "p", "-p", "f", "-o", "3"], 5.0 * 1024 * 10 * 4096),
+    }
+    runs = {}
+    for name, (args, flops) in cfgs.items():
+        runs[name] = {}
+        for t in threads:
+            t0 = time.perf_counter()
+            sec = _ref_bench_time(args + ["-n", str(t), "-i", "10"])
+            runs[name][str(t)] = {"min_ms": round(sec * 1e3, 4), "gflops": round(flops / sec / 1e9, 2)}
+            log(f"[bench] cpu reference {name} threads={t}: {sec * 1e3:.3f} ms "
+                f"({flops / sec / 1e9:.1f} GFLOP/s; {time.perf_counter() - t0:.1f} s)")
+    head = runs["2p24_fp64_inplace"]
+    best_t = max(head, key=lambda k: head[k]["gflops"])
+    base = {"value": head[best_t]["gflops"], "unit": "GFLOP/s", "cores": int(best_t), "kind": "reference",
+            "sample": (f"reference aocl_fftz_bench (built from its sources, OpenMP) "
+                       f"'16777216 -b p -p d -o 3 -r i -n {best_t} -i 10': min of 10 calibrated rounds "
+                       f"of the N=2^24 complex128 in-place forward transform, {head[best_t]['min_ms']} ms; "
+                       f"1 thread: {head['1']['min_ms']} ms ({head['1']['gflops']} GFLOP/s); "
+                       f"host affinity {avail} CPUs, OMP_NUM_THREADS={omp or 'unset'}")}
+    return base, {"cpu_reference_runs": runs, "cpu_threads_tried": threads}
+
+
+def cpu_baseline_agreement(res_2p24, res_1024):
+    """CPU-baseline leg, second part: relative L2 between the GPU outputs and the reference library's own
+    transform of the same inputs (ctypes into oracle/_ref; not timed)."""
     import numpy as np
     from oracle import reflib
     if not reflib.available():
-        return None, {}
-    cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
-    cores = max(1, min(cores, os.cpu_count() or 1))
+        return {}
     n = 1 << 24
-    x = np.ascontiguousarray(res_2p24["_x0"])
-    buf = x.copy()
-    plan = reflib.RefPlan([(n, 1, 1)], [(1, n, n)], "d", -1, buf, buf, threads=cores)
-    t_end = time.perf_counter() + budget_s
-    times = []
-    first = True
-    while time.perf_counter() < t_end or len(times) < 2:
-        buf[:] = x
-        t0 = time.perf_counter()
-        plan.execute()
-        times.append(time.perf_counter() - t0)
-        if first:
-            ref_y = buf.copy()
-            first = False
+    buf = np.ascontiguousarray(res_2p24["_x0"]).copy()
+    plan = reflib.RefPlan([(n, 1, 1)], [(1, n, n)], "d", -1, buf, buf, threads=min(16, os.cpu_count() or 1))
+    plan.execute()
     plan.destroy()
-    t = min(times)
-    gflops = 5.0 * n * math.log2(n) / t / 1e9
     y = res_2p24["_y"]
-    rel = float(np.linalg.norm(y - ref_y) / np.linalg.norm(ref_y))
-    extra = {"rel_l2_vs_reference_2p24": rel}
-    # batched N=1024 vs the reference's FP64 transform of the upcast input
+    out = {"rel_l2_vs_reference_2p24": float(np.linalg.norm(y - buf) / np.linalg.norm(buf))}
     xb = res_1024["_x0"].astype(np.complex128)
-    yb = reflib.c2c(xb[:256], -1, "d", threads=cores)
-    extra["rel_l2_vs_reference_n1024"] = float(np.linalg.norm(res_1024["_y"][:256] - yb) / np.linalg.norm(yb))
-    t_f, reps = reflib.time_c2c(1024, 4096, "f", cores, budget_s=min(5.0, budget_s / 2))
-    extra["cpu_n1024_gflops"] = 5.0 * 1024 * 10 * 4096 / t_f / 1e9
-    base = {"value": gflops, "unit": "GFLOP/s", "cores": cores, "kind": "reference",
-            "sample": f"{len(times)} forward executes of the N=2^24 complex128 in-place transform "
-                      f"(min time {t * 1e3:.1f} ms), AOCL-FFTZ built from its sources, OpenMP {cores} threads; "
-                      f"N=1024 x 4096 complex64: {reps} executes"}
-    return base, extra
+    yb = reflib.c2c(xb[:256], -1, "d", threads=min(16, os.cpu_count() or 1))
+    out["rel_l2_vs_reference_n1024"] = float(np.linalg.norm(res_1024["_y"][:256] - yb) / np.linalg.norm(yb))
+    return out
 
 
 def main():
@@ -581,17 +769,31 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--cpu-budget", type=float, default=12.0)
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the reference CPU baseline")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM-traffic runs")
+    ap.add_argument("--traffic-probe", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--legs", default="mixed,r2c,uneven,split,serving",
                     help="extra workloads after the primary and N=1024 legs (comma list; '' for none)")
     args = ap.parse_args()
+    if args.traffic_probe:
+        traffic_probe(args.traffic_probe)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the transforms here "
               "are single-GPU by design and no multi-GPU figure is produced.", file=sys.stderr)
         sys.exit(2)
     if args.steps + args.warmup > 40:
         sys.exit("steps + warmup must stay <= 40 (unnormalised round trips grow by N per step)")
+    # CPU baseline and counter runs are child processes started before this
+    # process initialises the GPU
+    base, extra = (None, {})
+    if not args.no_cpu:
+        log("[bench] cpu baseline: the reference's own benchmark program")
+        base, extra = cpu_baseline()
+    traffic, traffic_note = None, "not measured (--no-traffic)"
+    if not args.no_traffic:
+        traffic, traffic_note = measure_traffic()
+        log(f"[bench] traffic: {traffic_note}")
     import torch
     import fftz
     assert torch.cuda.is_available()
@@ -627,11 +829,25 @@ def main():
         rv = bench_serving(torch, fftz)
     copy_bps = copy_ceiling(torch)
     log(f"[bench] copy ceiling {copy_bps / 1e9:.0f} GB/s")
-    base, extra = (None, {})
     if not args.no_cpu:
-        log("[bench] cpu baseline (reference, bounded sample)")
-        base, extra = cpu_baseline(r24, r10, args.cpu_budget)
-    achieved = r24["dominant_achieved_Bps"]
+        extra.update(cpu_baseline_agreement(r24, r10))
+    # roofline of the headline: the whole four-step transform is the unit (its
+    # launches run back to back on one stream); algorithmic bytes = every
+    # element read once and written once = 2 x 16 B x 2^24 per transform
+    t_s = r24["transform_ms_avg"] * 1e-3
+    achieved = r24["transform_bytes"] / t_s
+    passes = []
+    for j, (b_j, ms_j) in enumerate(zip(r24["launch_bytes"], r24["launch_ms_avg"])):
+        pj = {"launch": j, "kernel": r24["launch_kernels"][j], "avg_us": round(ms_j * 1e3, 2),
+              "moved_GBps": round(b_j / (ms_j * 1e-3) / 1e9, 1)}
+        if traffic and "2p24" in traffic:
+            t = traffic["2p24"][j]
+            pj["traffic_read"] = int(t["read"])
+            pj["traffic_write"] = int(t["write"])
+        passes.append(pj)
+    tr = None
+    if traffic and "2p24" in traffic:
+        tr = int(sum(t["read"] + t["write"] for t in traffic["2p24"]))
     line = {
         "metric": "C2C FFT GFLOP/s (5Nlog2N) + %HBM roofline; MFMA-split vs plain at same error",
         "value": round(r24["gflops"], 2),
@@ -648,10 +864,13 @@ def main():
         "config": {"workload": "1-D C2C complex128 N=2^24 batch=1 in-place, forward+backward per step "
                                "(four-step path)", "n": 1 << 24, "batch": 1, "transforms_per_step": 2},
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4),
-                     "traffic": _traffic(r24["dominant_kernel"]),
-                     "kernel": f"{r24['dominant_kernel']}, launch #{r24['dominant_launch']} of the four-step plan",
-                     "algorithmic_bytes_per_launch": r24["launch_bytes"][r24["dominant_launch"]]},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": tr,
+                     "kernel": f"one four-step transform ({len(r24['launch_bytes'])} launches of "
+                               f"{', '.join(sorted(set(r24['launch_kernels'])))}), "
+                               f"event-timed average {r24['transform_ms_avg'] * 1e3:.1f} us",
+                     "algorithmic_bytes_per_transform": int(r24["transform_bytes"]),
+                     "traffic_source": traffic_note,
+                     "passes": passes},
         "cpu_baseline": base,
         "roundtrip_rel_l2": r24["roundtrip_rel_l2"],
         "transform_frac_of_hbm": round(r24["transform_frac_of_hbm"], 4),
@@ -664,6 +883,9 @@ def main():
         "split_mfma_n4096_b16384": rs,
         "serving": rv,
     }
+    if traffic and "n1024" in traffic:
+        t = traffic["n1024"][0]
+        line["batched_n1024_fp32"]["traffic"] = int(t["read"] + t["write"])
     line["copy_ceiling_GBps"] = round(copy_bps / 1e9, 1)
     line.update(extra)
     print(json.dumps(line), flush=True)

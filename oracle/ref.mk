@@ -1,5 +1,7 @@
 # Builds the reference AOCL-FFTZ CPU library from its own sources, where they
-# lie, into oracle/_ref/ (never into the reference tree).  The object list and
+# lie, into oracle/_ref/ (never into the reference tree), and the reference's
+# own benchmark program (test/bench, test/utils/dims_vecs_helper.c; target
+# aocl_fftz_bench of CMakeLists.txt:589-600) linked against that library.  The object list and
 # per-directory ISA flags follow the reference's CMakeLists.txt:70-175 (source
 # globs) and :200-242 (AVX flags), with ENABLE_MULTI_THREADING=ON (:247-265).
 # Nothing from the reference is copied into this repository; the resulting
@@ -26,7 +28,11 @@ isa_flags = $(if $(findstring /avx512/,$1),-mavx512f -mavx512dq -ffp-contract=fa
             $(if $(findstring /avx128/,$1),-mavx -mfma -ffp-contract=fast,\
             $(if $(findstring /kernels/,$1),-fno-version-loops-for-strides,))))
 
-all: $(OUT)/libaocl_fftz_ref.so
+BENCH_SRCS := $(sort $(wildcard $(REF)/test/bench/*.c $(REF)/test/bench/utils/*.c)) $(REF)/test/utils/dims_vecs_helper.c
+BENCH_OBJS := $(patsubst $(REF)/%.c,$(OBJDIR)/%.o,$(BENCH_SRCS))
+BENCH_INC  := -I$(REF)/test/bench -I$(REF)/test/bench/utils -I$(REF)/test/utils
+
+all: $(OUT)/libaocl_fftz_ref.so $(OUT)/aocl_fftz_bench
 
 $(OBJDIR)/%.o: $(REF)/%.c
 	@mkdir -p $(dir $@)
@@ -36,6 +42,15 @@ $(OBJDIR)/%.o: $(REF)/%.c
 # inside itself when a process also loads the GPU library (same names).
 $(OUT)/libaocl_fftz_ref.so: $(OBJS)
 	$(CC) -shared -fopenmp -Wl,-Bsymbolic -o $@ $(OBJS) -lm
+
+# the benchmark: timing build (ACCURACY_WITH_DFT off), finds the library
+# beside itself
+$(OBJDIR)/test/%.o: $(REF)/test/%.c
+	@mkdir -p $(dir $@)
+	@$(CC) $(BASE) $(BENCH_INC) -c $< -o $@
+
+$(OUT)/aocl_fftz_bench: $(BENCH_OBJS) $(OUT)/libaocl_fftz_ref.so
+	$(CC) -fopenmp -o $@ $(BENCH_OBJS) -L$(OUT) -laocl_fftz_ref -Wl,-rpath,'$$ORIGIN' -lm -ldl
 
 clean:
 	rm -rf $(OUT)
