@@ -36,6 +36,7 @@ __device__ __forceinline__ uint32_t fdiv(uint32_t n, const FastDiv& f)
 struct LineIdx {
     int64_t ioff, ooff;
     uint32_t l0;
+    int64_t gin, gout;  // sequence index of the line's element 0 (kChirpFlags)
 };
 
 // The buffers and line set a pass body works on: the descriptor's own for a
@@ -63,6 +64,11 @@ __device__ __forceinline__ LineIdx line_index(const PassDesc& p, const TileIo& i
     li.ioff = (int64_t)l0 * io.ils0 + (int64_t)l1 * p.ils[1] + (int64_t)l2 * p.ils[2];
     li.ooff = (int64_t)l0 * io.ols0 + (int64_t)l1 * p.ols[1] + (int64_t)l2 * p.ols[2];
     li.l0 = l0;
+    li.gin = li.gout = 0;
+    if (p.flags & kChirpFlags) {
+        li.gin = p.gin_base + (int64_t)l0 * p.gin[0] + (int64_t)l1 * p.gin[1] + (int64_t)l2 * p.gin[2];
+        li.gout = p.gout_base + (int64_t)l0 * p.gout[0] + (int64_t)l1 * p.gout[1] + (int64_t)l2 * p.gout[2];
+    }
     return li;
 }
 
@@ -423,29 +429,44 @@ struct LineIo {
     bool live_in, live_out;
 };
 
+// Element e of a line whose element 0 has sequence index gbase, as loaded:
+// with kChirpIn, (conjugated if backward) x * chirp[g] for g < nchirp and 0
+// beyond (the input holds only nchirp elements); else x for e < nvalid_in.
+template <typename R>
+__device__ __forceinline__ cplx<R> load_elem(const PassDesc& p, const cplx<R>* src, int e, int64_t gbase)
+{
+    cplx<R> v = {R(0), R(0)};
+    if (p.flags & kChirpIn) {
+        const int64_t g = gbase + (int64_t)e * p.gin[3];
+        if (g < p.nchirp) {
+            v = src[(int64_t)e * p.ies];
+            if (p.flags & kConjIn) v.y = -v.y;
+            v = cmul(v, reinterpret_cast<const cplx<R>*>(p.chirp)[g]);
+        }
+    } else if (e < p.nvalid_in) {
+        v = src[(int64_t)e * p.ies];
+    }
+    return v;
+}
+
 template <typename R, int E, int RAD>
 __device__ __forceinline__ void first_stage_direct(const PassDesc& p, cplx<R>* lds, int c, int t, const cplx<R>* src,
-                                                   bool live)
+                                                   bool live, int64_t gbase)
 {
     constexpr int K = (E + RAD - 1) / RAD;
     const int nbl = p.L / RAD;
-    const int nv = p.nvalid_in;
-    const int64_t ies = p.ies;
     cplx<R> v[K][RAD];
 #pragma unroll
     for (int k = 0; k < K; k++) {
         const int j = t + k * p.TL;
         if (live && j < nbl) {
 #pragma unroll
-            for (int i = 0; i < RAD; i++) {
-                const int e = i * nbl + j;
-                v[k][i] = {R(0), R(0)};
-                if (e < nv) v[k][i] = src[(int64_t)e * ies];
-            }
+            for (int i = 0; i < RAD; i++) v[k][i] = load_elem<R>(p, src, i * nbl + j, gbase);
         }
     }
     cplx<R>* line = lds + c * p.LP;
-    const bool cj = (p.flags & kConjIn) != 0;
+    // (with kChirpIn the conjugation happened before the chirp product)
+    const bool cj = (p.flags & (kConjIn | kChirpIn)) == kConjIn;
 #pragma unroll
     for (int k = 0; k < K; k++) {
         const int j = t + k * p.TL;
@@ -464,7 +485,7 @@ __device__ __forceinline__ void first_stage_direct(const PassDesc& p, cplx<R>* l
 // Output side of the last stage for one butterfly: element e = i*nbl + j.
 template <typename R, int RAD>
 __device__ __forceinline__ void store_butterfly(const PassDesc& p, cplx<R>* v, int j, int nbl, cplx<R>* dst,
-                                                uint32_t l0)
+                                                uint32_t l0, int64_t gbase)
 {
     if (p.flags & kTw4) {
         // w_N^(l0 e), e = i*nbl + j: two table products, then an FP64
@@ -484,9 +505,30 @@ __device__ __forceinline__ void store_butterfly(const PassDesc& p, cplx<R>* v, i
 #pragma unroll
         for (int i = 0; i < RAD; i++) v[i] = cscale(v[i], sc);
     }
+    if (p.flags & (kSpecOut | kChirpOut)) {
+        // multi-pass Bluestein: Z = conj(A) * spec after the first M-point
+        // transform; conj(chirp) * W after the second (stored for g < n only)
+#pragma unroll
+        for (int i = 0; i < RAD; i++) {
+            const int64_t g = gbase + (int64_t)(i * nbl + j) * p.gout[3];
+            if (p.flags & kSpecOut) {
+                v[i] = cmul(cconj(v[i]), reinterpret_cast<const cplx<R>*>(p.spec)[g]);
+            } else if (g < p.nchirp) {
+                v[i] = cmul(cconj(reinterpret_cast<const cplx<R>*>(p.chirp)[g]), v[i]);
+            }
+        }
+    }
     if (p.flags & kConjOut) {
 #pragma unroll
         for (int i = 0; i < RAD; i++) v[i].y = -v[i].y;
+    }
+    if (p.flags & kChirpOut) {
+#pragma unroll
+        for (int i = 0; i < RAD; i++) {
+            const int e = i * nbl + j;
+            if (gbase + (int64_t)e * p.gout[3] < p.nchirp) dst[(int64_t)e * p.oes] = v[i];
+        }
+        return;
     }
     const int nvo = p.nvalid_out;
     if (p.flags & kRealOut) {
@@ -510,7 +552,7 @@ __device__ __forceinline__ void store_butterfly(const PassDesc& p, cplx<R>* v, i
 
 template <typename R, int E, int RAD>
 __device__ __forceinline__ void last_stage_direct(const PassDesc& p, const cplx<R>* lds, int c, int t, cplx<R>* dst,
-                                                  uint32_t l0, bool live, int s)
+                                                  uint32_t l0, bool live, int s, int64_t gbase)
 {
     constexpr int K = (E + RAD - 1) / RAD;
     const int nbl = p.L / RAD;  // = ls of the last stage; g = 0, cc = j
@@ -535,7 +577,7 @@ __device__ __forceinline__ void last_stage_direct(const PassDesc& p, const cplx<
                 for (int i = 1; i < RAD; i++) v[k][i] = cmul(v[k][i], w[i - 1]);
             }
             Dft<RAD, R>::run(v[k]);
-            store_butterfly<R, RAD>(p, v[k], j, nbl, dst, l0);
+            store_butterfly<R, RAD>(p, v[k], j, nbl, dst, l0, gbase);
         }
     }
 }
@@ -543,18 +585,18 @@ __device__ __forceinline__ void last_stage_direct(const PassDesc& p, const cplx<
 // One-stage lengths (L <= 16): load, DFT, store, no LDS.
 template <typename R, int E, int RAD>
 __device__ __forceinline__ void single_stage_direct(const PassDesc& p, int t, const cplx<R>* src, cplx<R>* dst,
-                                                    uint32_t l0, bool live)
+                                                    uint32_t l0, bool live, int64_t gin, int64_t gout)
 {
     if (!live || t != 0) return;
     cplx<R> v[RAD];
+    const bool cj = (p.flags & (kConjIn | kChirpIn)) == kConjIn;
 #pragma unroll
     for (int i = 0; i < RAD; i++) {
-        v[i] = {R(0), R(0)};
-        if (i < p.nvalid_in) v[i] = src[(int64_t)i * p.ies];
-        if (p.flags & kConjIn) v[i].y = -v[i].y;
+        v[i] = load_elem<R>(p, src, i, gin);
+        if (cj) v[i].y = -v[i].y;
     }
     Dft<RAD, R>::run(v);
-    store_butterfly<R, RAD>(p, v, 0, 1, dst, l0);
+    store_butterfly<R, RAD>(p, v, 0, 1, dst, l0, gout);
 }
 
 #define FFTZ_RADIX_SWITCH(r, ALLRAD, CALL)                 \
@@ -592,32 +634,42 @@ __device__ __forceinline__ void pass_body_direct(const PassDesc& p, const TileIo
     const bool live_in = ci < p.C && (l_base + ci) < io.nlines;
     const bool live_out = co < p.C && (l_base + co) < io.nlines;
     const cplx<R>* src = nullptr;
-    if (live_in) src = reinterpret_cast<const cplx<R>*>(io.in) + line_index(p, io, (uint32_t)(l_base + ci)).ioff;
+    int64_t gin = 0;
+    if (live_in) {
+        LineIdx li = line_index(p, io, (uint32_t)(l_base + ci));
+        src = reinterpret_cast<const cplx<R>*>(io.in) + li.ioff;
+        gin = li.gin;
+    }
     cplx<R>* dst = nullptr;
     uint32_t l0 = 0;
+    int64_t gout = 0;
     if (live_out) {
         LineIdx li = line_index(p, io, (uint32_t)(l_base + co));
         dst = reinterpret_cast<cplx<R>*>(io.out) + li.ooff;
         l0 = li.l0;
+        gout = li.gout;
     }
     const int S = p.nst;
     if (S == 1) {
         // the loading thread stores: use the input-side line for both
         cplx<R>* d1 = nullptr;
         uint32_t l01 = 0;
+        int64_t gout1 = 0;
         if (live_in) {
             LineIdx li = line_index(p, io, (uint32_t)(l_base + ci));
             d1 = reinterpret_cast<cplx<R>*>(io.out) + li.ooff;
             l01 = li.l0;
+            gout1 = li.gout;
         }
-        FFTZ_RADIX_SWITCH(p.radix[0], ALLRAD, (single_stage_direct<R, E, RAD>(p, ti, src, d1, l01, live_in)))
+        FFTZ_RADIX_SWITCH(p.radix[0], ALLRAD,
+                          (single_stage_direct<R, E, RAD>(p, ti, src, d1, l01, live_in, gin, gout1)))
         return;
     }
-    FFTZ_RADIX_SWITCH(p.radix[0], ALLRAD, (first_stage_direct<R, E, RAD>(p, lds, ci, ti, src, live_in)))
+    FFTZ_RADIX_SWITCH(p.radix[0], ALLRAD, (first_stage_direct<R, E, RAD>(p, lds, ci, ti, src, live_in, gin)))
     __syncthreads();
     run_stages<R, ALLRAD, E>(p, lds, tid, 1, S - 1);
     FFTZ_RADIX_SWITCH(p.radix[S - 1], ALLRAD,
-                      (last_stage_direct<R, E, RAD>(p, lds, co, to, dst, l0, live_out, S - 1)))
+                      (last_stage_direct<R, E, RAD>(p, lds, co, to, dst, l0, live_out, S - 1, gout)))
 }
 
 template <typename R, bool ALLRAD, int E>

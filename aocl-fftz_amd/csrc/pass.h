@@ -39,7 +39,16 @@ enum PassFlags : uint32_t {
     kRealPromote = 1u << 11,  // R2C (any length): real element j loaded as (x[j], 0)
     kHermIn = 1u << 12,       // C2R (any length): bins 0..n/2 stored, bin k > n/2 is conj(bin n-k)
     kRealPart = 1u << 13,     // C2R (any length): store only the real part, as real elements
+    // Bluestein over several passes (transform length M above one LDS tile):
+    // element-wise products fused into the first / last pass of the two
+    // M-point four-step transforms.  g is the element's index in the whole
+    // M-point sequence (gin / gout below).
+    kChirpIn = 1u << 14,   // load: x[g] * chirp[g] for g < nchirp, 0 beyond
+    kSpecOut = 1u << 15,   // store: conj(y[g]) * spec[g]
+    kChirpOut = 1u << 16,  // store: conj(chirp[g]) * y[g], only g < nchirp
 };
+// Flags of the multi-pass Bluestein products (generic kernel only).
+constexpr uint32_t kChirpFlags = kChirpIn | kSpecOut | kChirpOut;
 // Flags whose passes stage the tile through LDS before the first stage.
 constexpr uint32_t kStagedFlags = kRealIn | kC2RPre | kR2CPost | kRealPromote | kHermIn | kRealPart;
 
@@ -81,6 +90,12 @@ struct PassDesc {
     int64_t ies, oes;      // element strides along the transform axis
     int64_t ils[3], ols[3];// line strides for line levels l0, l1, l2
     int64_t nlines;        // total lines
+    // index of an element in the whole sequence of a multi-pass transform:
+    // g = gbase + l0 * g[0] + l1 * g[1] + l2 * g[2] + e * g[3]
+    // (input side gin, output side gout; only read with kChirpFlags)
+    int64_t gin[4], gout[4];
+    int64_t gin_base, gout_base;
+    int64_t nchirp;        // Bluestein: n (valid input / output elements)
     int64_t tw4_n;         // four-step twiddle base N
     FastDiv ln0, ln1;      // line level counts l0, l1 (l2 = rest)
     int L;                 // FFT length of this pass

@@ -259,6 +259,9 @@ static int64_t fourstep_twiddles(TableBuilder& tb, int64_t N, int* lobits)
 // ---------------------------------------------------------------------------
 struct LineAxis {
     int64_t n, is, os;
+    // coefficients of this axis's index in the sequence index g of the
+    // input / output side (multi-pass Bluestein products; 0 for batch axes)
+    int64_t gi = 0, go = 0;
 };
 
 struct PassReq {
@@ -273,6 +276,9 @@ struct PassReq {
     int64_t tw4_n = 0;
     int64_t chirp_off = -1, spec_off = -1, rtw_off = -1;
     int extra_lds_elems = 0;   // C2R needs L+1 slots
+    int64_t in_base = 0, out_base = 0;  // byte offsets added to the role pointers
+    int64_t gie = 0, goe = 0;  // element coefficients of the sequence index (kChirpFlags)
+    int64_t nchirp = 0;
 };
 
 static int env_int(const char* name, int dflt)
@@ -291,7 +297,8 @@ static std::vector<LineAxis> merge_axes(std::vector<LineAxis> ax)
         for (size_t i = 0; i < ax.size() && !changed; i++)
             for (size_t j = 0; j < ax.size() && !changed; j++) {
                 if (i == j) continue;
-                if (ax[j].is == ax[i].is * ax[i].n && ax[j].os == ax[i].os * ax[i].n) {
+                if (ax[j].is == ax[i].is * ax[i].n && ax[j].os == ax[i].os * ax[i].n &&
+                    ax[j].gi == ax[i].gi * ax[i].n && ax[j].go == ax[i].go * ax[i].n) {
                     ax[i].n *= ax[j].n;
                     ax.erase(ax.begin() + (long)j);
                     changed = true;
@@ -414,9 +421,13 @@ static bool build_pass(Plan* plan, TableBuilder& tb, const PassReq& q, std::stri
 
     // fixed-length kernel: power-of-two L in [64, 4096], plain C2C pass
     const bool pow2L = q.L >= 64 && q.L <= 4096 && (q.L & (q.L - 1)) == 0;
+    if ((q.flags & kChirpFlags) && (q.kind != KernelKind::Pass || (q.flags & (kStagedFlags | kRealOut)))) {
+        err = "internal: Bluestein products on a staged pass";
+        return false;
+    }
     const bool fixed_ok = q.kind == KernelKind::Pass && pow2L && E == 16 && d.nvalid_in == q.L &&
                           d.nvalid_out == q.L &&
-                          !(q.flags & (kStagedFlags | kRealOut)) && env_int("FFTZ_FIXED", 1);
+                          !(q.flags & (kStagedFlags | kRealOut | kChirpFlags)) && env_int("FFTZ_FIXED", 1);
     bool plan_match = fixed_ok;
     if (fixed_ok) {
         // the fixed kernels hard-code 16, 16, ..., 2^rem (kernels_fixed.hip)
@@ -430,7 +441,8 @@ static bool build_pass(Plan* plan, TableBuilder& tb, const PassReq& q, std::stri
     // plan, row mapping, no fused twiddle / real-packed step
     const int mm = plan->spec.mfma_mode;
     const bool use_mfma = mm != kMfmaNone && prec == 4 && plan_match && (q.L == 256 || q.L == 4096) &&
-                          !q.want_in_col && !q.want_out_col && !(q.flags & (kTw4 | kStagedFlags | kRealOut));
+                          !q.want_in_col && !q.want_out_col &&
+                          !(q.flags & (kTw4 | kStagedFlags | kRealOut | kChirpFlags));
 
     // line axes: choose l0
     std::vector<LineAxis> ax = merge_axes(q.lines);
@@ -511,7 +523,12 @@ static bool build_pass(Plan* plan, TableBuilder& tb, const PassReq& q, std::stri
     for (int k = 0; k < 3; k++) {
         d.ils[k] = ax[k].is;
         d.ols[k] = ax[k].os;
+        d.gin[k] = ax[k].gi;
+        d.gout[k] = ax[k].go;
     }
+    d.gin[3] = q.gie;
+    d.gout[3] = q.goe;
+    d.nchirp = q.nchirp;
     if (q.flags & kTw4) {
         d.tw4_n = q.tw4_n;
         ln.tw4_off = fourstep_twiddles(tb, q.tw4_n, &d.tw4_lobits);
@@ -539,25 +556,31 @@ static bool build_pass(Plan* plan, TableBuilder& tb, const PassReq& q, std::stri
     if (out_col && !in_col && tiles >= 64 && env_int("FFTZ_XCD_REMAP", 1)) d.flags |= kXcdRemap;
 
     // expand host-looped axes
-    std::vector<int64_t> ioffs{0}, ooffs{0};
+    std::vector<int64_t> ioffs{q.in_base}, ooffs{q.out_base}, gis{0}, gos{0};
     // real-side strides count real elements (packed R2C / C2R and the
     // full-length promoted / real-part forms alike)
     const int64_t in_scale = ((q.flags & (kRealIn | kRealPromote)) ? prec : esize);
     const int64_t out_scale = ((q.flags & (kRealOut | kRealPart)) ? prec : esize);
     for (auto& h : host_axes) {
-        std::vector<int64_t> ni, no;
+        std::vector<int64_t> ni, no, ngi, ngo;
         for (size_t k = 0; k < ioffs.size(); k++)
             for (int64_t j = 0; j < h.n; j++) {
                 ni.push_back(ioffs[k] + j * h.is * in_scale);
                 no.push_back(ooffs[k] + j * h.os * out_scale);
+                ngi.push_back(gis[k] + j * h.gi);
+                ngo.push_back(gos[k] + j * h.go);
             }
         ioffs.swap(ni);
         ooffs.swap(no);
+        gis.swap(ngi);
+        gos.swap(ngo);
     }
     for (size_t k = 0; k < ioffs.size(); k++) {
         Launch l2 = ln;
         l2.in_off_bytes = ioffs[k];
         l2.out_off_bytes = ooffs[k];
+        l2.d.gin_base = gis[k];
+        l2.d.gout_base = gos[k];
         plan->launches.push_back(l2);
     }
     return true;
@@ -573,7 +596,43 @@ static int64_t bluestein_length(int64_t n)
         if (smooth13(m)) return m;
 }
 
-static void bluestein_tables(TableBuilder& tb, int prec, int64_t n, int64_t M, int64_t* chirp_off, int64_t* spec_off)
+Plan* plan_create(const ProblemSpec& spec, std::string& err, bool dry_run);
+hipError_t plan_enqueue(Plan* p, Slot* s, const void* in, void* out, hipStream_t stream);
+
+// In-place FP64 forward FFT of v (13-smooth length) on the device, through a
+// plan of this library: the spectrum table of a long Bluestein transform
+// (M up to ~2^26), where the host recursion would take tens of seconds.
+static bool device_fft_fp64(std::vector<d2>& v)
+{
+    ProblemSpec ss;
+    ss.prec_bytes = 8;
+    ss.type = FftType::C2C;
+    ss.dir = 0;
+    ss.inplace = true;
+    ss.dims = {{(int64_t)v.size(), 1, 1}};
+    std::string err;
+    Plan* sub = plan_create(ss, err, false);
+    if (!sub) return false;
+    void* buf = nullptr;
+    const size_t bytes = v.size() * sizeof(d2);
+    bool ok = hipMalloc(&buf, bytes) == hipSuccess &&
+              hipMemcpy(buf, v.data(), bytes, hipMemcpyHostToDevice) == hipSuccess;
+    if (ok) {
+        hipError_t e = hipSuccess;
+        Slot* s = plan_acquire_slot(sub, &e);
+        ok = s && plan_enqueue(sub, s, buf, buf, s->stream) == hipSuccess &&
+             hipStreamSynchronize(s->stream) == hipSuccess;
+        if (s) plan_release_slot(sub, s);
+    }
+    ok = ok && hipMemcpy(v.data(), buf, bytes, hipMemcpyDeviceToHost) == hipSuccess;
+    if (buf) (void)hipFree(buf);
+    plan_destroy(sub);
+    return ok;
+}
+
+// dry_run: only the table sizes matter (no device, zero spectrum).
+static void bluestein_tables(TableBuilder& tb, int prec, int64_t n, int64_t M, int64_t* chirp_off, int64_t* spec_off,
+                             bool dry_run)
 {
     std::vector<d2> b((size_t)M, d2{0, 0});
     std::vector<d2> cb((size_t)n);
@@ -585,8 +644,15 @@ static void bluestein_tables(TableBuilder& tb, int prec, int64_t n, int64_t M, i
         b[(size_t)k] = bk;
         if (k > 0) b[(size_t)(M - k)] = bk;
     }
-    std::vector<d2> fb((size_t)M);
-    host_fft_rec(M, b.data(), 1, fb.data());
+    std::vector<d2> fb;
+    if (dry_run) {
+        fb.assign((size_t)M, d2{0, 0});
+    } else if (M > (1 << 18) && device_fft_fp64(b)) {
+        fb.swap(b);
+    } else {
+        fb.resize((size_t)M);
+        host_fft_rec(M, b.data(), 1, fb.data());
+    }
     const double inv = 1.0 / (double)M;
     for (auto& v : fb) v = {v.x * inv, -v.y * inv};
     *chirp_off = tb.add_complex(cb, prec);
@@ -611,53 +677,6 @@ static int64_t extent(const std::vector<AxisSpec>& dims, const std::vector<AxisS
     }
     for (auto& v : vecs) e += (v.n - 1) * std::llabs(use_in ? v.is : v.os);
     return e + 1;
-}
-
-// One complete 1-D transform along axis `ax` of a tensor as a list of passes.
-// `first`: the pass reads the caller's input geometry; otherwise it reads and
-// writes the output buffer in place.
-static bool plan_axis_c2c(Plan* plan, TableBuilder& tb, const std::vector<AxisSpec>& dims,
-                          const std::vector<AxisSpec>& vecs, size_t axis, bool first, bool conj_in, bool conj_out,
-                          Role in_role, Role out_role, bool use_out_strides_for_in, std::string& err)
-{
-    const int prec = plan->spec.prec_bytes;
-    const int64_t n = dims[axis].n;
-    PassReq q;
-    q.in_role = in_role;
-    q.out_role = out_role;
-    q.ies = use_out_strides_for_in ? dims[axis].os : dims[axis].is;
-    q.oes = dims[axis].os;
-    for (size_t j = 0; j < dims.size(); j++)
-        if (j != axis)
-            q.lines.push_back({dims[j].n, use_out_strides_for_in ? dims[j].os : dims[j].is, dims[j].os});
-    for (auto& v : vecs) q.lines.push_back({v.n, use_out_strides_for_in ? v.os : v.is, v.os});
-    (void)first;
-    if (conj_in) q.flags |= kConjIn;
-    if (conj_out) q.flags |= kConjOut;
-    q.want_in_col = q.ies != 1;
-    q.want_out_col = q.oes != 1;
-    if (smooth13(n)) {
-        if (n > single_pass_limit(prec)) {
-            err = "axis length " + std::to_string(n) + " above the single-pass limit in a multi-axis problem";
-            return false;
-        }
-        q.L = (int)n;
-        return build_pass(plan, tb, q, err);
-    }
-    // Bluestein
-    const int64_t M = bluestein_length(n);
-    if (M > single_pass_limit(prec)) {
-        err = "Bluestein length " + std::to_string(M) + " for prime factor length " + std::to_string(n) +
-              " exceeds the fused single-pass limit";
-        return false;
-    }
-    q.kind = KernelKind::Bluestein;
-    q.L = (int)M;
-    q.nvalid_in = (int)n;
-    q.nvalid_out = (int)n;
-    bluestein_tables(tb, prec, n, M, &q.chirp_off, &q.spec_off);
-    q.want_in_col = q.want_out_col = false;
-    return build_pass(plan, tb, q, err);
 }
 
 // Balanced factorisation of a large 1-D length into p factors, each <= fmax
@@ -710,14 +729,31 @@ static bool split_factors(int64_t N, int64_t fmax, std::vector<int64_t>& f)
     return false;
 }
 
-// Four-step over factors f[0..p-1] (p = 2 or 3): see pow2_fourstep_solver_dft.c
-// for the p = 2 form.  Step 0 reads the caller's input; the last step writes
-// the caller's output with the transpose folded into its store.
-static bool plan_large_1d(Plan* plan, TableBuilder& tb, const AxisSpec& dim, const std::vector<AxisSpec>& vecs,
-                          bool conj_in, bool conj_out, std::string& err)
+// A length-N transform along one axis of a tensor, through the work buffer,
+// as a four-step over p = 2 or 3 factors (the p = 2 form is
+// core/solvers/complex/pow2_fourstep_solver_dft.c:700-763, with the twiddle
+// and transpose of fused_twiddle_transpose_c.c:20-110 folded into the passes'
+// stores): the first pass reads the source and writes the dense intermediate
+// [batch][N] in the work buffer with the inter-pass twiddle applied, middle
+// passes work in place on it, and the last pass writes the destination with
+// the transpose folded into its store.  Any number of other axes ride along
+// as batch lines (the reference loops them, batched_solver_dft.c:33-110).
+struct LongAxis {
+    int64_t N = 0;
+    int64_t is = 1, os = 1;               // element strides in the source / destination
+    std::vector<AxisSpec> batch;          // other axes: n, source stride, destination stride
+    Role src = Role::In, dst = Role::Out;
+    int64_t src_base = 0, dst_base = 0;   // bytes into the source / destination buffer
+    int64_t work_base = 0;                // bytes into the work buffer of the intermediate
+    bool conj_in = false, conj_out = false;
+    uint32_t first_flags = 0, last_flags = 0;  // Bluestein products (kChirpFlags)
+    int64_t nchirp = 0, chirp_off = -1, spec_off = -1;
+};
+
+static bool plan_long_axis(Plan* plan, TableBuilder& tb, const LongAxis& a, std::string& err)
 {
     const int prec = plan->spec.prec_bytes;
-    const int64_t N = dim.n;
+    const int64_t N = a.N;
     // Largest pass length: 256 keeps column tiles 16 lines wide (256-byte
     // FP64 / 128-byte FP32 segments) with 256-thread workgroups; grow it only
     // when three passes of 256 cannot cover N.
@@ -731,16 +767,16 @@ static bool plan_large_1d(Plan* plan, TableBuilder& tb, const AxisSpec& dim, con
     // power-of-two factors last: the final (transposing) pass then runs on
     // the fixed-length kernel
     if (!getenv("FFTZ_FACTORS"))
-        std::stable_sort(f.begin(), f.end(), [](int64_t a, int64_t b) {
-            return ((a & (a - 1)) != 0) > ((b & (b - 1)) != 0);
+        std::stable_sort(f.begin(), f.end(), [](int64_t x, int64_t y) {
+            return ((x & (x - 1)) != 0) > ((y & (y - 1)) != 0);
         });
-    std::vector<AxisSpec> bv = vecs;
-    if (bv.size() > 1) {
-        err = "large 1-D transform with more than one batch axis";
-        return false;
+    // dense intermediate: batch axis i at stride N * prod_{j<i} n_j
+    std::vector<int64_t> wst(a.batch.size());
+    int64_t B = 1;
+    for (size_t i = 0; i < a.batch.size(); i++) {
+        wst[i] = N * B;
+        B *= a.batch[i].n;
     }
-    int64_t B = bv.empty() ? 1 : bv[0].n;
-    int64_t bis = bv.empty() ? 0 : bv[0].is, bos = bv.empty() ? 0 : bv[0].os;
     const size_t p = f.size();
     std::vector<int64_t> M(p);  // M[s] = N / (f0...fs)
     int64_t acc = N;
@@ -748,21 +784,28 @@ static bool plan_large_1d(Plan* plan, TableBuilder& tb, const AxisSpec& dim, con
         acc /= f[s];
         M[s] = acc;
     }
-    plan->work_bytes = std::max(plan->work_bytes, (size_t)(N * B * 2 * prec));
-    // step 0
+    plan->work_bytes = std::max(plan->work_bytes, (size_t)(a.work_base + N * B * 2 * prec));
+    // step 0: columns of the source (element g = l0 + e M0), twiddle on store
     {
         PassReq q;
         q.L = (int)f[0];
-        q.ies = M[0] * dim.is;
+        q.ies = M[0] * a.is;
         q.oes = M[0];
-        q.lines.push_back({M[0], dim.is, 1});
-        if (B > 1) q.lines.push_back({B, bis, N});
-        q.flags = kTw4 | (conj_in ? kConjIn : 0);
+        LineAxis l0{M[0], a.is, 1};
+        l0.gi = 1;
+        q.lines.push_back(l0);
+        for (size_t i = 0; i < a.batch.size(); i++) q.lines.push_back({a.batch[i].n, a.batch[i].is, wst[i]});
+        q.flags = kTw4 | (a.conj_in ? kConjIn : 0) | (a.first_flags & kChirpIn);
         q.tw4_n = N;
-        q.want_in_col = (dim.is == 1);
+        q.want_in_col = (a.is == 1);
         q.want_out_col = true;
-        q.in_role = Role::In;
+        q.in_role = a.src;
         q.out_role = Role::Work;
+        q.in_base = a.src_base;
+        q.out_base = a.work_base;
+        q.gie = M[0];
+        q.nchirp = a.nchirp;
+        q.chirp_off = a.chirp_off;
         if (!build_pass(plan, tb, q, err)) return false;
     }
     // middle steps (p == 3)
@@ -772,32 +815,44 @@ static bool plan_large_1d(Plan* plan, TableBuilder& tb, const AxisSpec& dim, con
         q.ies = M[s];
         q.oes = M[s];
         q.lines.push_back({M[s], 1, 1});
-        int64_t rows = N / M[s - 1];
+        const int64_t rows = N / M[s - 1];
         q.lines.push_back({rows, M[s - 1], M[s - 1]});
-        if (B > 1) q.lines.push_back({B, N, N});
+        for (size_t i = 0; i < a.batch.size(); i++) q.lines.push_back({a.batch[i].n, wst[i], wst[i]});
         q.flags = kTw4;
         q.tw4_n = M[s - 1];
         q.want_in_col = q.want_out_col = true;
-        q.in_role = Role::Work;
-        q.out_role = Role::Work;
+        q.in_role = q.out_role = Role::Work;
+        q.in_base = q.out_base = a.work_base;
         if (!build_pass(plan, tb, q, err)) return false;
     }
-    // last step: rows of length f[p-1], transposed store
+    // last step: rows of length f[p-1]; row (k0, k1) element e is output
+    // k = k0 + f0 k1 + e (N / f[p-1]), stored at k * os
     {
         PassReq q;
         q.L = (int)f[p - 1];
         q.ies = 1;
-        int64_t outer = N / f[p - 1];
-        q.oes = dim.os * outer;
-        // row (k0, k1): input offset k0*M0 + k1*M1, output (k0 + f0 k1) * os
-        q.lines.push_back({f[0], M[0], dim.os});
-        if (p == 3) q.lines.push_back({f[1], M[1], dim.os * f[0]});
-        if (B > 1) q.lines.push_back({B, N, bos});
-        q.flags = (conj_out ? kConjOut : 0);
+        const int64_t outer = N / f[p - 1];
+        q.oes = a.os * outer;
+        q.goe = outer;
+        LineAxis k0{f[0], M[0], a.os};
+        k0.go = 1;
+        q.lines.push_back(k0);
+        if (p == 3) {
+            LineAxis k1{f[1], M[1], a.os * f[0]};
+            k1.go = f[0];
+            q.lines.push_back(k1);
+        }
+        for (size_t i = 0; i < a.batch.size(); i++) q.lines.push_back({a.batch[i].n, wst[i], a.batch[i].os});
+        q.flags = (a.conj_out ? kConjOut : 0) | (a.last_flags & (kSpecOut | kChirpOut));
         q.want_in_col = false;
-        q.want_out_col = (dim.os == 1);
+        q.want_out_col = (a.os == 1);
         q.in_role = Role::Work;
-        q.out_role = Role::Out;
+        q.out_role = a.dst;
+        q.in_base = a.work_base;
+        q.out_base = a.dst_base;
+        q.nchirp = a.nchirp;
+        q.chirp_off = a.chirp_off;
+        q.spec_off = a.spec_off;
         if (!build_pass(plan, tb, q, err)) return false;
     }
     std::ostringstream os;
@@ -807,17 +862,126 @@ static bool plan_large_1d(Plan* plan, TableBuilder& tb, const AxisSpec& dim, con
     return true;
 }
 
+// Bluestein for a length n whose extended length M exceeds one LDS tile
+// (core/solvers/complex/bluestein_solver_dft.c:137-234): the chirp product
+// x conj(B) rides on the first pass of a four-step FFT_M, the spectrum
+// product conj(A) conj(FFT_M(b)) / M on its last store, a second four-step
+// FFT_M reads that and its last store forms conj(B W) (forward) or B W
+// (backward, whose input was conjugated on load) for the n outputs.  Work
+// buffer: [intermediate of both transforms | product Z], M x batch each.
+static bool plan_bluestein_long(Plan* plan, TableBuilder& tb, int64_t n, int64_t is, int64_t os,
+                                const std::vector<AxisSpec>& batch, Role src, Role dst, bool bwd,
+                                std::string& err)
+{
+    const int prec = plan->spec.prec_bytes;
+    const int64_t M = bluestein_length(n);
+    int64_t B = 1;
+    for (auto& b : batch) B *= b.n;
+    int64_t chirp_off = -1, spec_off = -1;
+    bluestein_tables(tb, prec, n, M, &chirp_off, &spec_off, plan->dry_run);
+    const int64_t zbase = M * B * 2 * prec;  // region of the product Z
+    std::vector<AxisSpec> b1 = batch, b2 = batch;
+    int64_t dense = M;
+    for (size_t i = 0; i < batch.size(); i++) {
+        b1[i].os = dense;
+        b2[i].is = dense;
+        dense *= batch[i].n;
+    }
+    LongAxis a1;
+    a1.N = M;
+    a1.is = is;
+    a1.os = 1;
+    a1.batch = b1;
+    a1.src = src;
+    a1.dst = Role::Work;
+    a1.dst_base = zbase;
+    a1.conj_in = bwd;
+    a1.first_flags = kChirpIn;
+    a1.last_flags = kSpecOut;
+    a1.nchirp = n;
+    a1.chirp_off = chirp_off;
+    a1.spec_off = spec_off;
+    if (!plan_long_axis(plan, tb, a1, err)) return false;
+    LongAxis a2;
+    a2.N = M;
+    a2.is = 1;
+    a2.os = os;
+    a2.batch = b2;
+    a2.src = Role::Work;
+    a2.src_base = zbase;
+    a2.dst = dst;
+    a2.conj_out = !bwd;
+    a2.last_flags = kChirpOut;
+    a2.nchirp = n;
+    a2.chirp_off = chirp_off;
+    if (!plan_long_axis(plan, tb, a2, err)) return false;
+    plan->work_bytes = std::max(plan->work_bytes, (size_t)(2 * zbase));
+    plan->description += "Bluestein n=" + std::to_string(n) + " M=" + std::to_string(M) + " (multi-pass); ";
+    return true;
+}
+
+// One complete 1-D transform along axis `ax` of a tensor as a list of passes:
+// one LDS pass when the length fits a tile (a fused Bluestein pass for a
+// length with a prime factor above 13), else the four-step or multi-pass
+// Bluestein through the work buffer.  The input is read with the caller's
+// input strides, or with the output strides when the axis runs in place on
+// the output (use_out_strides_for_in).
+static bool plan_axis_c2c(Plan* plan, TableBuilder& tb, const std::vector<AxisSpec>& dims,
+                          const std::vector<AxisSpec>& vecs, size_t axis, bool conj_in, bool conj_out,
+                          Role in_role, Role out_role, bool use_out_strides_for_in, std::string& err)
+{
+    const int prec = plan->spec.prec_bytes;
+    const int64_t n = dims[axis].n;
+    PassReq q;
+    q.in_role = in_role;
+    q.out_role = out_role;
+    q.ies = use_out_strides_for_in ? dims[axis].os : dims[axis].is;
+    q.oes = dims[axis].os;
+    std::vector<AxisSpec> batch;
+    for (size_t j = 0; j < dims.size(); j++)
+        if (j != axis) batch.push_back({dims[j].n, use_out_strides_for_in ? dims[j].os : dims[j].is, dims[j].os});
+    for (auto& v : vecs) batch.push_back({v.n, use_out_strides_for_in ? v.os : v.is, v.os});
+    for (auto& b : batch) q.lines.push_back({b.n, b.is, b.os});
+    if (conj_in) q.flags |= kConjIn;
+    if (conj_out) q.flags |= kConjOut;
+    q.want_in_col = q.ies != 1;
+    q.want_out_col = q.oes != 1;
+    if (smooth13(n)) {
+        if (n > single_pass_limit(prec)) {
+            LongAxis a;
+            a.N = n;
+            a.is = q.ies;
+            a.os = q.oes;
+            a.batch = batch;
+            a.src = in_role;
+            a.dst = out_role;
+            a.conj_in = conj_in;
+            a.conj_out = conj_out;
+            return plan_long_axis(plan, tb, a, err);
+        }
+        q.L = (int)n;
+        return build_pass(plan, tb, q, err);
+    }
+    // Bluestein
+    const int64_t M = bluestein_length(n);
+    if (M > single_pass_limit(prec))
+        return plan_bluestein_long(plan, tb, n, q.ies, q.oes, batch, in_role, out_role, conj_in, err);
+    q.kind = KernelKind::Bluestein;
+    q.L = (int)M;
+    q.nvalid_in = (int)n;
+    q.nvalid_out = (int)n;
+    bluestein_tables(tb, prec, n, M, &q.chirp_off, &q.spec_off, plan->dry_run);
+    q.want_in_col = q.want_out_col = false;
+    return build_pass(plan, tb, q, err);
+}
+
 static bool plan_c2c(Plan* plan, TableBuilder& tb, std::vector<AxisSpec> dims, const std::vector<AxisSpec>& vecs,
                      std::string& err)
 {
-    const int prec = plan->spec.prec_bytes;
     const bool bwd = plan->spec.dir == 1;
-    if (dims.size() == 1 && smooth13(dims[0].n) && dims[0].n > single_pass_limit(prec))
-        return plan_large_1d(plan, tb, dims[0], vecs, bwd, bwd, err);
     for (size_t a = 0; a < dims.size(); a++) {
         const bool first = (a == 0);
-        if (!plan_axis_c2c(plan, tb, dims, vecs, a, first, bwd, bwd, first ? Role::In : Role::Out, Role::Out,
-                           !first, err))
+        if (!plan_axis_c2c(plan, tb, dims, vecs, a, bwd, bwd, first ? Role::In : Role::Out, Role::Out, !first, err))
             return false;
     }
     plan->description += "row-column " + std::to_string(dims.size()) + "-D; ";
@@ -852,7 +1016,7 @@ static bool real_full_pass(Plan* plan, TableBuilder& tb, PassReq q, int64_t n, s
     }
     q.kind = KernelKind::Bluestein;
     q.L = (int)M;
-    bluestein_tables(tb, prec, n, M, &q.chirp_off, &q.spec_off);
+    bluestein_tables(tb, prec, n, M, &q.chirp_off, &q.spec_off, plan->dry_run);
     return build_pass(plan, tb, q, err);
 }
 
@@ -897,7 +1061,7 @@ static bool plan_real(Plan* plan, TableBuilder& tb, std::vector<AxisSpec> dims, 
         cd[0].n = nbins;
         for (size_t a = 1; a < dims.size(); a++) {
             std::vector<AxisSpec> od = cd;
-            if (!plan_axis_c2c(plan, tb, od, vecs, a, false, false, false, Role::Out, Role::Out, true, err))
+            if (!plan_axis_c2c(plan, tb, od, vecs, a, false, false, Role::Out, Role::Out, true, err))
                 return false;
         }
         plan->description += packed ? "real-packed R2C; " : "real R2C (full length); ";
@@ -927,7 +1091,7 @@ static bool plan_real(Plan* plan, TableBuilder& tb, std::vector<AxisSpec> dims, 
         for (size_t a = 1; a < dims.size(); a++) {
             const bool first = (a == 1);
             std::vector<AxisSpec> od = wdims;
-            if (!plan_axis_c2c(plan, tb, od, wvecs, a, first, true, true, first ? Role::In : Role::Work,
+            if (!plan_axis_c2c(plan, tb, od, wvecs, a, true, true, first ? Role::In : Role::Work,
                                Role::Work, !first, err))
                 return false;
         }
@@ -1030,7 +1194,7 @@ static bool batch_size_desc(TableBuilder& tb, int prec, int64_t n, int E, int ma
     d.stw = (const void*)(intptr_t)stw;
     if (si.bluestein) {
         int64_t co, so;
-        bluestein_tables(tb, prec, n, L, &co, &so);
+        bluestein_tables(tb, prec, n, L, &co, &so, false);
         d.chirp = (const void*)(intptr_t)co;
         d.spec = (const void*)(intptr_t)so;
     }
@@ -1041,6 +1205,7 @@ Plan* batch_plan_create(const std::vector<BatchItem>& items, int prec, int dir, 
                         const TuneParams* tune)
 {
     Plan* plan = new Plan();
+    plan->dry_run = dry_run;
     plan->spec.prec_bytes = prec;
     plan->spec.dir = dir;
     plan->spec.type = FftType::C2C;
@@ -1332,6 +1497,7 @@ int tune_import(const char* path)
 Plan* plan_create(const ProblemSpec& spec_in, std::string& err, bool dry_run)
 {
     Plan* plan = new Plan();
+    plan->dry_run = dry_run;
     plan->spec = spec_in;
     ProblemSpec& spec = plan->spec;
     if (spec.tune.empty()) (void)tune_lookup(problem_key(spec), spec.tune);
@@ -1461,7 +1627,10 @@ std::string plan_json(const Plan* p)
            << ",\"E\":" << ln.E << ",\"fixed\":" << (ln.fixed ? 1 : 0) << ",\"mfma\":" << ln.mfma << ",\"grid\":" << ln.grid << ",\"block\":" << ln.block
            << ",\"lds\":" << ln.lds_bytes << ",\"in_role\":" << (int)ln.in_role
            << ",\"out_role\":" << (int)ln.out_role << ",\"in_off\":" << ln.in_off_bytes
-           << ",\"out_off\":" << ln.out_off_bytes << ",\"tw4_n\":" << d.tw4_n << ",\"radix\":[";
+           << ",\"out_off\":" << ln.out_off_bytes << ",\"tw4_n\":" << d.tw4_n << ",\"nchirp\":" << d.nchirp
+           << ",\"gin\":[" << d.gin[0] << "," << d.gin[1] << "," << d.gin[2] << "," << d.gin[3] << "],\"gout\":["
+           << d.gout[0] << "," << d.gout[1] << "," << d.gout[2] << "," << d.gout[3] << "],\"gin_base\":" << d.gin_base
+           << ",\"gout_base\":" << d.gout_base << ",\"radix\":[";
         for (int s = 0; s < d.nst; s++) os << (s ? "," : "") << d.radix[s];
         os << "],\"ls\":[";
         for (int s = 0; s < d.nst; s++) os << (s ? "," : "") << d.ls[s];
@@ -1547,15 +1716,19 @@ static hipError_t record_prof(Plan* p, hipStream_t stream)
     return hipEventRecord(ev, stream);
 }
 
-static hipError_t launch_batch(Plan* p, Slot* s, const void* in, void* out, hipStream_t stream)
+// Launches of one execute.  work: the work buffer (four-step and multi-axis
+// C2R intermediates), counters: the uneven batch's queue counters; prof:
+// record timing events (which allocates them on first use).
+static hipError_t launch_batch(Plan* p, unsigned* counters, const void* in, void* out, hipStream_t stream,
+                               bool prof)
 {
     // counters are cleared by a kernel (not a memset node) so the sequence
     // can be captured in a graph and replayed
     int nc = (int)p->bclasses.size();
-    void* rargs[] = {&s->counters, &nc};
+    void* rargs[] = {&counters, &nc};
     hipError_t e = hipLaunchKernel(kernel_symbol_reset(), dim3(1), dim3(64), rargs, 0, stream);
     if (e != hipSuccess) return e;
-    if (p->profile && (e = record_prof(p, stream)) != hipSuccess) return e;
+    if (prof && (e = record_prof(p, stream)) != hipSuccess) return e;
     const char* base = (const char*)p->tables;
     for (size_t c = 0; c < p->bclasses.size(); c++) {
         const BatchClass& bc = p->bclasses[c];
@@ -1564,7 +1737,7 @@ static hipError_t launch_batch(Plan* p, Slot* s, const void* in, void* out, hipS
             a.descs = reinterpret_cast<const PassDesc*>(base + p->descs_off);
             a.tiles = reinterpret_cast<const BatchTile*>(base + bc.tiles_off);
             a.ntiles = bc.ntiles;
-            a.counter = s->counters + c;
+            a.counter = counters + c;
             a.in = in;
             a.out = out;
             void* args[] = {&a};
@@ -1572,15 +1745,16 @@ static hipError_t launch_batch(Plan* p, Slot* s, const void* in, void* out, hipS
                                 dim3(bc.maxt), args, bc.lds_bytes, stream);
             if (e != hipSuccess) return e;
         }
-        if (p->profile && (e = record_prof(p, stream)) != hipSuccess) return e;
+        if (prof && (e = record_prof(p, stream)) != hipSuccess) return e;
     }
     return hipGetLastError();
 }
 
-static hipError_t launch_all(Plan* p, Slot* s, const void* in, void* out, hipStream_t stream)
+static hipError_t launch_all(Plan* p, void* work, unsigned* counters, const void* in, void* out, hipStream_t stream,
+                             bool prof)
 {
-    if (p->is_batch) return launch_batch(p, s, in, out, stream);
-    if (p->profile) {
+    if (p->is_batch) return launch_batch(p, counters, in, out, stream, prof);
+    if (prof) {
         hipError_t e = record_prof(p, stream);
         if (e != hipSuccess) return e;
     }
@@ -1590,7 +1764,7 @@ static hipError_t launch_all(Plan* p, Slot* s, const void* in, void* out, hipStr
             switch (r) {
                 case Role::In: return (char*)in;
                 case Role::Out: return (char*)out;
-                default: return (char*)s->work;
+                default: return (char*)work;
             }
         };
         d.in = role_ptr(ln.in_role) + ln.in_off_bytes;
@@ -1603,12 +1777,19 @@ static hipError_t launch_all(Plan* p, Slot* s, const void* in, void* out, hipStr
         hipError_t e = hipLaunchKernel(fn, dim3(ln.grid),
                                        dim3(ln.block), args, ln.lds_bytes, stream);
         if (e != hipSuccess) return e;
-        if (p->profile) {
+        if (prof) {
             e = record_prof(p, stream);
             if (e != hipSuccess) return e;
         }
     }
     return hipGetLastError();
+}
+
+hipError_t plan_enqueue_scratch(Plan* p, void* work, unsigned* counters, const void* in, void* out,
+                                hipStream_t stream)
+{
+    if (p->noop) return hipSuccess;
+    return launch_all(p, work, counters, in, out, stream, false);
 }
 
 int plan_profile_read(Plan* p, float* ms, int cap)
@@ -1638,7 +1819,7 @@ int plan_profile_read(Plan* p, float* ms, int cap)
 hipError_t plan_enqueue(Plan* p, Slot* s, const void* in, void* out, hipStream_t stream)
 {
     if (p->noop) return hipSuccess;
-    if (!p->use_graph || p->profile) return launch_all(p, s, in, out, stream);
+    if (!p->use_graph || p->profile) return launch_all(p, s->work, s->counters, in, out, stream, p->profile);
     // One hipGraph per (slot, in, out): captured once, replayed afterwards.
     if (!s->graph || s->graph_in != in || s->graph_out != out) {
         if (s->graph) {
@@ -1648,7 +1829,7 @@ hipError_t plan_enqueue(Plan* p, Slot* s, const void* in, void* out, hipStream_t
         hipGraph_t g;
         hipError_t e = hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal);
         if (e != hipSuccess) return e;
-        e = launch_all(p, s, in, out, s->stream);
+        e = launch_all(p, s->work, s->counters, in, out, s->stream, false);
         hipError_t e2 = hipStreamEndCapture(s->stream, &g);
         if (e != hipSuccess) return e;
         if (e2 != hipSuccess) return e2;

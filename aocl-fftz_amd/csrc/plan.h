@@ -102,6 +102,7 @@ struct Plan {
     ProblemSpec spec;
     int device = 0;
     bool noop = false;
+    bool dry_run = false;  // launch list only: no device memory, zero-valued device-built tables
     std::vector<Launch> launches;
     size_t work_bytes = 0;
     int64_t in_extent_bytes = 0, out_extent_bytes = 0;
@@ -164,6 +165,12 @@ void plan_destroy(Plan* p);
 // Enqueue the plan on the slot's stream (or the user stream) for device
 // buffers.  Returns hipSuccess or the first HIP error.
 hipError_t plan_enqueue(Plan* p, Slot* s, const void* in, void* out, hipStream_t stream);
+
+// Enqueue the plan's kernels on `stream` with caller-provided work memory
+// (four-step intermediates, plan->work_bytes) and queue counters (uneven
+// batch, one per tile class): no slot, graph, event or allocation.
+hipError_t plan_enqueue_scratch(Plan* p, void* work, unsigned* counters, const void* in, void* out,
+                                hipStream_t stream);
 
 // Read back per-launch durations (ms) recorded since the last read, in
 // launch order (execute-major).  Returns the number of durations.

@@ -14,6 +14,22 @@ import numpy as np
 K_CONJ_IN, K_CONJ_OUT, K_TW4 = 1, 2, 4
 K_SCALE, K_R2C_POST, K_C2R_PRE, K_REAL_IN, K_REAL_OUT = 32, 128, 256, 512, 1024
 K_REAL_PROMOTE, K_HERM_IN, K_REAL_PART = 2048, 4096, 8192
+K_CHIRP_IN, K_SPEC_OUT, K_CHIRP_OUT = 1 << 14, 1 << 15, 1 << 16
+
+
+def chirp_conj(n):
+    """conj(B_k) = exp(-i pi (k^2 mod 2n) / n), k < n."""
+    k = np.arange(n, dtype=np.int64)
+    return np.exp(-1j * np.pi * ((k * k) % (2 * n)) / n)
+
+
+def bluestein_spec(n, m):
+    """conj(FFT_m(b)) / m with b_k = B_k, b_{m-k} = B_k (k < n), zero elsewhere."""
+    b = np.zeros(m, np.complex128)
+    bk = np.conj(chirp_conj(n))
+    b[:n] = bk
+    b[m - np.arange(1, n)] = bk[1:]
+    return np.conj(np.fft.fft(b)) / m
 
 
 class OOB(AssertionError):
@@ -28,7 +44,15 @@ def _line_offsets(ln, nl):
     l2 = r // ln["ln1"]
     ioff = l0 * ln["ils"][0] + l1 * ln["ils"][1] + l2 * ln["ils"][2]
     ooff = l0 * ln["ols"][0] + l1 * ln["ols"][1] + l2 * ln["ols"][2]
-    return ioff, ooff, l0
+    return ioff, ooff, l0, l1, l2
+
+
+def _seq_base(ln, key, l0, l1, l2):
+    """Sequence index of each line's element 0 (multi-pass Bluestein products)."""
+    g = ln.get(key)
+    if g is None:
+        return np.zeros_like(l0)
+    return ln[key + "_base"] + l0 * g[0] + l1 * g[1] + l2 * g[2]
 
 
 def _check(idx, size, what):
@@ -45,7 +69,7 @@ def run(plan, bufs):
         L = ln["L"]
         flags = ln["flags"]
         nl = ln["nlines"]
-        ioff, ooff, l0 = _line_offsets(ln, nl)
+        ioff, ooff, l0, l1, l2 = _line_offsets(ln, nl)
         # launch offsets are bytes in the plan's precision: complex elements
         # are 2 * prec bytes, real elements prec bytes
         prec = plan["prec"]
@@ -79,6 +103,20 @@ def run(plan, bufs):
             x = np.zeros((nl, max(L, n)), np.complex128)
             v = src[idx]
             x[:, :n] = np.where((2 * e > n)[None, :], np.conj(v), v)
+        elif flags & K_CHIRP_IN:
+            # x[g] conj(B_g) for sequence index g < nchirp, zero beyond (only
+            # the valid elements are read)
+            n = ln["nchirp"]
+            e = np.arange(L, dtype=np.int64)
+            g = _seq_base(ln, "gin", l0, l1, l2)[:, None] + e[None, :] * ln["gin"][3]
+            valid = g < n
+            idx = ioff[:, None] + e[None, :] * ln["ies"]
+            _check(idx[valid], src.size, "chirp load")
+            x = np.zeros((nl, L), np.complex128)
+            x[valid] = src[idx[valid]]
+            if flags & K_CONJ_IN:
+                x = np.conj(x)
+            x[valid] *= chirp_conj(n)[g[valid]]
         else:
             nv = ln["nvalid_in"]
             e = np.arange(nv, dtype=np.int64)
@@ -86,7 +124,7 @@ def run(plan, bufs):
             _check(idx, src.size, "load")
             x = np.zeros((nl, max(L, nv)), np.complex128)
             x[:, :nv] = src[idx]
-        if flags & K_CONJ_IN:
+        if flags & K_CONJ_IN and not flags & K_CHIRP_IN:
             x = np.conj(x)
         if ln["kind"] == "bluestein":
             n = ln["nvalid_in"]
@@ -126,9 +164,23 @@ def run(plan, bufs):
             p = (l0[:, None] * e[None, :])
             assert p.max() < N, "four-step twiddle exponent exceeds its table"
             y = y * np.exp(-2j * np.pi * p / N)
+        if flags & (K_SPEC_OUT | K_CHIRP_OUT):
+            g = _seq_base(ln, "gout", l0, l1, l2)[:, None] + e[None, :] * ln["gout"][3]
+            n = ln["nchirp"]
+            if flags & K_SPEC_OUT:
+                m = L * ln["gout"][3]  # the sequence length: pass length x outer count
+                assert g.max() < m
+                y = np.conj(y) * bluestein_spec(n, m)[g]
+            else:
+                y = np.where(g < n, np.conj(chirp_conj(n))[np.minimum(g, n - 1)] * y, y)
         if flags & K_CONJ_OUT:
             y = np.conj(y)
-        if flags & K_REAL_PART:
+        if flags & K_CHIRP_OUT:
+            valid = g < ln["nchirp"]
+            idx = ooff[:, None] + e[None, :] * ln["oes"]
+            _check(idx[valid], dst.size, "chirp store")
+            dst[idx[valid]] = y[valid]
+        elif flags & K_REAL_PART:
             idx = ooff[:, None] + e[None, :] * ln["oes"]
             _check(idx, dst.size, "real-part store")
             dst[idx] = y.real

@@ -63,8 +63,7 @@ PRIMES = [17, 19, 23, 31, 101, 199, 1009, 4099, 8191]
 @pytest.mark.parametrize("n", PRIMES)
 @pytest.mark.parametrize("prec", ["f", "d"])
 def test_bluestein(torch_gpu, n, prec):
-    if prec == "d" and n > 4096:
-        pytest.skip("FP64 Bluestein length above the fused single-pass limit")
+    # FP64 4099 / 8191 (extended length 8232 / 16384) run the multi-pass form
     x = rand_c((3, n), n, prec)
     y, info = run_c2c(torch_gpu, x)
     assert "bluestein" in info

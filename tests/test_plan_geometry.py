@@ -34,10 +34,43 @@ def c2c_case(dims, vecs, prec="f", direction=0, inplace=False):
 @pytest.mark.parametrize("n", [1, 2, 16, 17, 64, 100, 101, 1024, 4096, 8191, 16384, 12288])
 @pytest.mark.parametrize("prec", ["f", "d"])
 def test_1d_batched(n, prec):
-    if prec == "d" and n == 8191:
-        pytest.skip("FP64 Bluestein above single-pass limit")
     b = 3
     c2c_case([(n, 1, 1)], [(b, n, n)], prec)
+
+
+@pytest.mark.parametrize("n,prec,direction", [(16411, "f", 0), (16411, "f", 1), (8191, "d", 0), (4099, "d", 1),
+                                              (65537, "d", 0)])
+def test_bluestein_multi_pass(n, prec, direction):
+    """Extended length M above one LDS tile: chirp product on the first pass
+    of a four-step FFT_M, spectrum product on its last store, a second FFT_M
+    whose last store forms the n outputs."""
+    plan = c2c_case([(n, 1, 1)], [(2, n, n)], prec, direction)
+    assert sum(1 for ln in plan["launches"] if ln["flags"] & emu.K_CHIRP_IN) >= 1
+    assert sum(1 for ln in plan["launches"] if ln["flags"] & emu.K_CHIRP_OUT) >= 1
+
+
+def test_bluestein_multi_pass_strided_inplace():
+    # element stride 2 (interleaved with a second batch), in place, backward
+    c2c_case([(16411, 2, 2)], [(2, 1, 1)], "f", 1, inplace=True)
+    # the Bluestein axis is the outer axis of a 2-D problem
+    c2c_case([(5, 1, 1), (16411, 5, 5)], [(2, 5 * 16411, 5 * 16411)], "f")
+    c2c_case([(7, 1, 1), (65537, 7, 7)], [(1, 1, 1)], "d", 0, inplace=True)
+
+
+@pytest.mark.parametrize("dims,vecs,prec", [
+    ([(8, 1, 1), (32768, 8, 8)], [(1, 1, 1)], "f"),                      # 2-D, long outer axis
+    ([(32768, 1, 1), (8, 32768, 32768)], [(1, 1, 1)], "f"),              # 2-D, long inner axis
+    ([(65536, 1, 1), (4, 65536, 65536), (4, 4 * 65536, 4 * 65536)], [(1, 1, 1)], "f"),  # 3-D
+    ([(20000, 1, 1), (3, 20000, 20000)], [(1, 1, 1)], "d"),              # FP64 long axis
+    ([(1 << 15, 1, 1)], [(3, 1 << 15, 1 << 15), (2, 3 << 15, 3 << 15)], "f"),  # two batch axes
+    ([(1 << 14, 1, 1)], [(3, 1 << 14, 1 << 14), (2, 3 << 14, 3 << 14), (2, 6 << 14, 6 << 14)], "d"),
+])
+def test_long_axis_generality(dims, vecs, prec):
+    """An axis above the single-pass limit in an N-D problem, and a long 1-D
+    transform with several batch axes: four-step with the other axes as
+    batch lines."""
+    c2c_case(dims, vecs, prec)
+    c2c_case(dims, vecs, prec, direction=1)
 
 
 @pytest.mark.parametrize("n,prec", [(1 << 15, "f"), (1 << 17, "d"), (1 << 20, "f"), (1 << 20, "d"),
