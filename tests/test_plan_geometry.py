@@ -224,3 +224,41 @@ def test_real_full_length_many_batch_axes(n, prec):
     emu.run(plan_b, {0: out.copy(), 1: back, 2: np.zeros(max(1, plan_b["work_bytes"] // 16), np.complex128)})
     for oi in _vec_offsets(vecs_r2c, 1):
         assert np.max(np.abs(back[oi:oi + n] / n - xr[oi:oi + n])) < 1e-10
+
+
+@pytest.mark.parametrize("kind", ["row_oop", "row_inplace", "col_oop", "col_inplace", "strided_oop"])
+@pytest.mark.parametrize("n", [50, 64, 15, 17, 1024])
+def test_real_layouts_emulated(kind, n):
+    """R2C then C2R in the reference's row / column-major, in / out-of-place
+    layouts (docs/rst/data_layout_conventions.rst:56-200)."""
+    import real_layouts as RL
+    b = 4
+    dims, vecs, inplace = RL.layout(kind, n, b)
+    nr, nc = RL.extents(dims, vecs, n)
+    plan = fftz.plan_json(dims, vecs, "d", 0, fft_type=1, inplace=inplace)
+    emu.check_launch_limits(plan)
+    rng = np.random.default_rng(n)
+    size_r = (max(nr, 2 * nc) + 1) // 2 * 2 if inplace else nr
+    buf_r = rng.standard_normal(size_r)
+    x = buf_r[RL.real_index(dims, vecs, n)].copy()
+    if inplace:
+        view = buf_r.view(np.complex128)  # the same memory, complex elements
+        emu.run(plan, {0: buf_r, 1: view, 2: np.zeros(1, np.complex128)})
+        out = view
+    else:
+        out = np.zeros(nc, np.complex128)
+        emu.run(plan, {0: buf_r, 1: out, 2: np.zeros(1, np.complex128)})
+    y = out[RL.cplx_index(dims, vecs, n)]
+    assert O.rel_l2(y, np.fft.rfft(x, axis=1)) < 1e-12
+    # C2R back
+    cd, cv = RL.c2r_of(dims, vecs)
+    plan_b = fftz.plan_json(cd, cv, "d", 1, fft_type=1, inplace=inplace)
+    emu.check_launch_limits(plan_b)
+    work = np.zeros(max(1, plan_b["work_bytes"] // 16), np.complex128)
+    if inplace:
+        emu.run(plan_b, {0: out, 1: buf_r, 2: work})
+        back = buf_r
+    else:
+        back = np.zeros(nr)
+        emu.run(plan_b, {0: out.copy(), 1: back, 2: work})
+    assert O.rel_l2(back[RL.real_index(dims, vecs, n)] / n, x) < 1e-12
