@@ -1,6 +1,8 @@
 """Build the MI355X FFT library in-tree with hipcc (gfx950 only).
 
     python aocl-fftz_amd/build.py            # -> aocl-fftz_amd/lib/libaocl_fftz.so
+    python aocl-fftz_amd/build.py --incremental   # recompile only sources newer
+                                                  # than their object (or any header)
 
 Every HIP source is compiled with packed-FP32 instruction generation
 disabled (-target-feature -packed-fp32-ops).
@@ -30,8 +32,17 @@ CFLAGS = [
 ]
 
 
-def _compile(src: str) -> str:
+def _headers_mtime() -> float:
+    hs = glob.glob(os.path.join(CSRC, "*.h")) + glob.glob(os.path.join(ROOT, "include", "*.h"))
+    return max(os.path.getmtime(h) for h in hs)
+
+
+def _compile(src: str, incremental: bool = False) -> str:
     obj = os.path.join(OBJ, os.path.basename(src) + ".o")
+    if incremental and os.path.exists(obj):
+        t = os.path.getmtime(obj)
+        if t > os.path.getmtime(src) and t > _headers_mtime():
+            return obj
     cmd = [HIPCC] + CFLAGS + ["-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -44,12 +55,12 @@ def _compile(src: str) -> str:
     return obj
 
 
-def build(jobs: int = 8) -> str:
+def build(jobs: int = 8, incremental: bool = False) -> str:
     os.makedirs(OBJ, exist_ok=True)
     os.makedirs(LIBDIR, exist_ok=True)
     srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")) + glob.glob(os.path.join(CSRC, "*.cpp")))
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(_compile, srcs))
+        objs = list(ex.map(lambda src: _compile(src, incremental), srcs))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -60,4 +71,4 @@ def build(jobs: int = 8) -> str:
 
 
 if __name__ == "__main__":
-    print(build())
+    print(build(incremental="--incremental" in sys.argv))
