@@ -15,6 +15,7 @@
 
 #include "../../include/aoclfftz.h"
 #include "../../include/aoclfftz_gpu.h"
+#include "noalloc.h"
 #include "plan.h"
 
 namespace {
@@ -695,6 +696,60 @@ aoclfftz_error_type aoclfftz_gpu_execute_many(FFTZ_VOID* handle, FFTZ_VOID** ins
         if (!ins[i] || !outs[i]) return AOCLFFTZ_EXECUTION_FAILURE;
         aoclfftz_error_type rc = run(h, ins[i], outs[i], "aoclfftz_gpu_execute_many");
         if (rc != AOCLFFTZ_SUCCESS) return rc;
+    }
+    return AOCLFFTZ_SUCCESS;
+}
+
+aoclfftz_error_type aoclfftz_gpu_execute_noalloc(FFTZ_VOID* handle, FFTZ_VOID* in, FFTZ_VOID* out)
+{
+    const char* fname = "aoclfftz_gpu_execute_noalloc";
+    Handle* h = as_handle(handle);
+    if (!h || in == nullptr || out == nullptr) return AOCLFFTZ_EXECUTION_FAILURE;
+    fftz::Plan* p = h->plan;
+    if (p->noop) return AOCLFFTZ_SUCCESS;
+    if (p->spec.inplace && in != out) {
+        report_error(fname, "in-place plan executed with distinct buffers");
+        return AOCLFFTZ_EXECUTION_FAILURE;
+    }
+    const size_t counter_bytes = p->is_batch ? p->bclasses.size() * sizeof(unsigned) : 0;
+    if (p->work_bytes > fftz::kNoAllocWorkBytes || counter_bytes > fftz::kNoAllocHeaderBytes) {
+        report_error(fname, "plan needs %zu bytes of work memory; the allocation-free execute holds at most %zu",
+                     p->work_bytes, fftz::kNoAllocWorkBytes);
+        return AOCLFFTZ_INVALID_INPUT;
+    }
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev != p->device) {
+        report_error(fname, "the current device (%d) is not the plan's device (%d)", dev, p->device);
+        return AOCLFFTZ_INVALID_INPUT;
+    }
+    void* scratch = nullptr;
+    hipError_t e = fftz::noalloc_scratch(dev, &scratch);
+    if (e != hipSuccess) {
+        report_error(fname, "device scratch lookup failed: %s", hipGetErrorString(e));
+        return AOCLFFTZ_EXECUTION_FAILURE;
+    }
+    unsigned* counters = reinterpret_cast<unsigned*>(scratch);
+    void* work = static_cast<unsigned char*>(scratch) + fftz::kNoAllocHeaderBytes;
+    std::lock_guard<std::mutex> g(h->amu);
+    if (h->stream) {
+        e = fftz::plan_enqueue_scratch(p, work, counters, in, out, h->stream);
+        if (e != hipSuccess) {
+            report_error(fname, "launch failed: %s", hipGetErrorString(e));
+            return AOCLFFTZ_EXECUTION_FAILURE;
+        }
+        return AOCLFFTZ_SUCCESS;
+    }
+    // synchronous form: device buffers only (staging host buffers would
+    // allocate), launched on the null stream and waited for
+    if (!is_device_ptr(in) || !is_device_ptr(out)) {
+        report_error(fname, "in and out must be device memory");
+        return AOCLFFTZ_INVALID_INPUT;
+    }
+    e = fftz::plan_enqueue_scratch(p, work, counters, in, out, nullptr);
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+    if (e != hipSuccess) {
+        report_error(fname, "execution failed: %s", hipGetErrorString(e));
+        return AOCLFFTZ_EXECUTION_FAILURE;
     }
     return AOCLFFTZ_SUCCESS;
 }

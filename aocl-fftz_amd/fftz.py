@@ -75,7 +75,7 @@ EXPORTS = (
     "aoclfftz_gpu_plan_info", "aoclfftz_gpu_plan_flops", "aoclfftz_gpu_plan_device_bytes",
     "aoclfftz_gpu_plan_json", "aoclfftz_gpu_profile", "aoclfftz_gpu_profile_read",
     "aoclfftz_gpu_plan_launches", "aoclfftz_gpu_execute_many", "aoclfftz_gpu_batch_setup",
-    "aoclfftz_gpu_batch_plan_json",
+    "aoclfftz_gpu_batch_plan_json", "aoclfftz_gpu_execute_noalloc",
 )
 
 
@@ -104,6 +104,8 @@ def load():
     lib.aoclfftz_execute.restype = ctypes.c_int
     lib.aoclfftz_execute_io.argtypes = [vp, vp, vp]
     lib.aoclfftz_execute_io.restype = ctypes.c_int
+    lib.aoclfftz_gpu_execute_noalloc.argtypes = [vp, vp, vp]
+    lib.aoclfftz_gpu_execute_noalloc.restype = ctypes.c_int
     lib.aoclfftz_destroy.argtypes = [vp]
     lib.aoclfftz_destroy.restype = None
     lib.aoclfftz_version.restype = ctypes.c_char_p
@@ -233,6 +235,10 @@ class Plan:
 
     def execute_io(self, inp, out) -> int:
         return load().aoclfftz_execute_io(self.handle, _ptr(inp), _ptr(out))
+
+    def execute_noalloc(self, inp, out) -> int:
+        """aoclfftz_gpu_execute_noalloc: device buffers, static device scratch."""
+        return load().aoclfftz_gpu_execute_noalloc(self.handle, _ptr(inp), _ptr(out))
 
     def execute_many(self, ins, outs) -> int:
         """Execute on each (ins[i], outs[i]) pair, in order, in one call."""

@@ -30,6 +30,27 @@ aoclfftz_error_type aoclfftz_gpu_set_stream(FFTZ_VOID *handle, FFTZ_VOID *stream
 aoclfftz_error_type aoclfftz_gpu_execute_many(FFTZ_VOID *handle, FFTZ_VOID **ins, FFTZ_VOID **outs,
                                              FFTZ_INT32 count);
 
+/* Execute the plan on (in, out) without allocating anything: no hipMalloc,
+ * hipFree, stream, event or graph is created by the call.  No counterpart in
+ * the reference; it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid allocation).
+ *
+ * in / out must be device memory.  Results are those of aoclfftz_execute_io
+ * on the same buffers (the same kernels run).  With a stream bound
+ * (aoclfftz_gpu_set_stream) the kernels are enqueued there and the call
+ * returns without waiting; otherwise they run on the null stream and the
+ * call waits for them.  Plans that need work memory (four-step and
+ * multi-pass transforms, uneven batches) use a static per-GPU scratch array
+ * of 64 MiB of work memory plus 256 bytes of queue counters, held by every
+ * GPU the library runs on until the process ends; a plan needing more
+ * returns AOCLFFTZ_INVALID_INPUT with a message and launches nothing.  The
+ * array is looked up on the first call on each device (with that device
+ * current) and must be the plan's device.  All calls on one GPU share its
+ * array and it keeps the last call's contents: calls that overlap on one GPU
+ * (from several threads, or on different streams) race and must be ordered
+ * by the caller. */
+aoclfftz_error_type aoclfftz_gpu_execute_noalloc(FFTZ_VOID *handle, FFTZ_VOID *in, FFTZ_VOID *out);
+
 /* Wait for all work the handle has enqueued. */
 aoclfftz_error_type aoclfftz_gpu_synchronize(FFTZ_VOID *handle);
 
