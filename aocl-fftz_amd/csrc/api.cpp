@@ -504,12 +504,12 @@ aoclfftz_error_type aoclfftz_gpu_autotune(FFTZ_VOID* handle, FFTZ_VOID* in, FFTZ
     }
     fftz::Plan* best_plan = nullptr;
     double best = 1e30;
-    for (const fftz::TuneParams& t : cands) {
+    auto trial = [&](const fftz::TuneParams& t) {
         fftz::ProblemSpec spec = p->spec;
         spec.tune = t;
         std::string err;
         fftz::Plan* q = fftz::plan_create(spec, err);
-        if (!q) continue;
+        if (!q) return;
         const double ms = fftz::plan_time(q, in, out, reps);
         if (ms >= 0 && ms < best) {
             if (best_plan) fftz::plan_destroy(best_plan);
@@ -518,8 +518,46 @@ aoclfftz_error_type aoclfftz_gpu_autotune(FFTZ_VOID* handle, FFTZ_VOID* in, FFTZ
         } else {
             fftz::plan_destroy(q);
         }
-    }
+    };
+    // 1. tile shapes
+    for (const fftz::TuneParams& t : cands) trial(t);
     if (!best_plan) return AOCLFFTZ_EXECUTION_FAILURE;
+    // 2. four-step split of the long axis (plans with a work buffer whose
+    //    problem is one long 1-D axis)
+    if (p->spec.dims.size() == 1 && p->work_bytes > 0 && p->spec.type == fftz::FftType::C2C) {
+        const fftz::TuneParams base = best_plan->spec.tune;
+        for (const auto& f : fftz::split_candidates(p->spec.dims[0].n, p->spec.prec_bytes, 6)) {
+            fftz::TuneParams t = base;
+            t.split = f;
+            trial(t);
+        }
+    }
+    // 3. radix order of each distinct pass length, one length at a time
+    {
+        std::vector<int> lengths;
+        for (const auto& ln : best_plan->launches)
+            if (ln.kind == fftz::KernelKind::Pass && ln.d.nst >= 2 && !ln.mfma &&
+                std::find(lengths.begin(), lengths.end(), ln.d.L) == lengths.end())
+                lengths.push_back(ln.d.L);
+        for (int L : lengths) {
+            const fftz::TuneParams base = best_plan->spec.tune;
+            const auto orders = fftz::radix_orders(L, 6);
+            for (size_t i = 1; i < orders.size(); i++) {  // [0] is the default order
+                fftz::TuneParams t = base;
+                t.radix_for.push_back({L, orders[i]});
+                trial(t);
+            }
+        }
+    }
+    // 4. complex values per thread
+    if (p->spec.mfma_mode == fftz::kMfmaNone) {
+        const fftz::TuneParams base = best_plan->spec.tune;
+        for (int e : {8, 16}) {
+            fftz::TuneParams t = base;
+            t.elems = e;
+            trial(t);
+        }
+    }
     fftz::ProblemSpec key_spec = p->spec;
     key_spec.tune = fftz::TuneParams{};
     fftz::tune_store(fftz::problem_key(key_spec), best_plan->spec.tune);

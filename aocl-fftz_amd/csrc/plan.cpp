@@ -28,6 +28,8 @@
 #include <functional>
 #include <map>
 #include <sstream>
+#include <thread>
+#include <atomic>
 
 namespace fftz {
 
@@ -106,9 +108,12 @@ static d2 root_fwd(int64_t p, int64_t n)
 
 // ---------------------------------------------------------------------------
 // Host FP64 FFT (recursive mixed radix) used only at setup, for the Bluestein
-// spectrum table.
+// spectrum table.  The roots of the top-level length are tabulated once
+// (root_fwd, exact integer reduction); a sub-transform of length m reads
+// root(j, m) as tab[j * (n / m)], which is the same double as root_fwd(j, m)
+// (both reduce to the same correctly rounded fraction of the circle).
 // ---------------------------------------------------------------------------
-static void host_fft_rec(int64_t n, const d2* in, int64_t is, d2* out)
+static void host_fft_rec(int64_t n, const d2* in, int64_t is, d2* out, const d2* tab, int64_t tstride)
 {
     if (n == 1) { out[0] = in[0]; return; }
     int64_t r = 0;
@@ -118,8 +123,8 @@ static void host_fft_rec(int64_t n, const d2* in, int64_t is, d2* out)
         for (int64_t k = 0; k < n; k++) {
             d2 s = {0, 0};
             for (int64_t j = 0; j < n; j++) {
-                d2 w = root_fwd(j * k, n);
-                d2 x = in[j * is];
+                const d2 w = tab[((j * k) % n) * tstride];
+                const d2 x = in[j * is];
                 s.x += x.x * w.x - x.y * w.y;
                 s.y += x.x * w.y + x.y * w.x;
             }
@@ -129,18 +134,25 @@ static void host_fft_rec(int64_t n, const d2* in, int64_t is, d2* out)
     }
     const int64_t m = n / r;
     std::vector<d2> y((size_t)n);
-    for (int64_t q = 0; q < r; q++) host_fft_rec(m, in + q * is, is * r, y.data() + q * m);
+    for (int64_t q = 0; q < r; q++) host_fft_rec(m, in + q * is, is * r, y.data() + q * m, tab, tstride * r);
     for (int64_t k1 = 0; k1 < m; k1++)
         for (int64_t k2 = 0; k2 < r; k2++) {
             d2 s = {0, 0};
             for (int64_t q = 0; q < r; q++) {
-                d2 w = root_fwd(q * (k1 + m * k2), n);
-                d2 x = y[(size_t)(q * m + k1)];
+                const d2 w = tab[((q * (k1 + m * k2)) % n) * tstride];
+                const d2 x = y[(size_t)(q * m + k1)];
                 s.x += x.x * w.x - x.y * w.y;
                 s.y += x.x * w.y + x.y * w.x;
             }
             out[k1 + m * k2] = s;
         }
+}
+
+static void host_fft(int64_t n, const d2* in, d2* out)
+{
+    std::vector<d2> tab((size_t)n);
+    for (int64_t j = 0; j < n; j++) tab[(size_t)j] = root_fwd(j, n);
+    host_fft_rec(n, in, 1, out, tab.data(), 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -182,6 +194,57 @@ static bool factor_radices(int L, std::vector<int>& out)
     }
     std::sort(out.begin(), out.end(), std::greater<int>());
     return (int)out.size() <= kMaxStages;
+}
+
+// A radix order the kernels can run: stages from kRadices whose product is L.
+static bool radix_order_ok(int L, const std::vector<int>& r)
+{
+    if (r.empty() || (int)r.size() > kMaxStages) return false;
+    int64_t prod = 1;
+    for (int v : r) {
+        if (std::find(std::begin(kRadices), std::end(kRadices), v) == std::end(kRadices)) return false;
+        prod *= v;
+    }
+    return prod == L;
+}
+
+std::vector<std::vector<int>> radix_orders(int L, int maxn)
+{
+    // Multisets of radices with product L and at most one stage more than
+    // the fewest; each in descending and ascending order.  The default order
+    // (factor_radices) comes first.
+    std::vector<std::vector<int>> out;
+    std::vector<int> dflt;
+    if (!factor_radices(L, dflt) || dflt.size() < 2) return out;
+    out.push_back(dflt);
+    const size_t smax = dflt.size() + 1;
+    std::vector<std::vector<int>> sets;
+    std::vector<int> cur;
+    std::function<void(int64_t, size_t)> rec = [&](int64_t n, size_t from) {
+        if (n == 1) {
+            if (cur.size() >= 2) sets.push_back(cur);
+            return;
+        }
+        if (cur.size() >= smax) return;
+        for (size_t i = from; i < sizeof(kRadices) / sizeof(kRadices[0]); i++) {
+            const int r = kRadices[i];
+            if (n % r) continue;
+            cur.push_back(r);
+            rec(n / r, i);
+            cur.pop_back();
+        }
+    };
+    rec(L, 0);
+    std::stable_sort(sets.begin(), sets.end(),
+                     [](const std::vector<int>& a, const std::vector<int>& b) { return a.size() < b.size(); });
+    for (const auto& m : sets) {
+        std::vector<int> d = m, a = m;
+        std::sort(d.begin(), d.end(), std::greater<int>());
+        std::sort(a.begin(), a.end());
+        for (const auto& o : {d, a})
+            if ((int)out.size() < maxn && std::find(out.begin(), out.end(), o) == out.end()) out.push_back(o);
+    }
+    return out;
 }
 
 static int next_pow2(int64_t v)
@@ -387,6 +450,10 @@ static bool build_pass(Plan* plan, TableBuilder& tb, const PassReq& q, std::stri
         err = "length " + std::to_string(q.L) + " has no radix plan";
         return false;
     }
+    // a tuned radix order for this length (plain passes; the matrix-core
+    // pass needs the all-16 plan)
+    if (const std::vector<int>* tr = plan->spec.tune.radices(q.L))
+        if (q.kind == KernelKind::Pass && plan->spec.mfma_mode == kMfmaNone && radix_order_ok(q.L, *tr)) radix = *tr;
     d.nst = (int)radix.size();
     ln.all_radix = false;
     for (size_t s = 0; s < radix.size(); s++) {
@@ -404,9 +471,9 @@ static bool build_pass(Plan* plan, TableBuilder& tb, const PassReq& q, std::stri
     // the 1024-thread variant within its 128 VGPRs (16 would spill)
     const bool fixed_range = q.L >= 64 && q.L <= 4096 && (q.L & (q.L - 1)) == 0;
     if (prec == 8 && !fixed_range && next_pow2((q.L + 7) / 8) <= 1024) E = 8;
-    if (env_int("FFTZ_E", 0) == 8 || env_int("FFTZ_E", 0) == 16) {
-        const int e = env_int("FFTZ_E", 0);
-        if (next_pow2((q.L + e - 1) / e) <= 1024) E = e;
+    {
+        const int e = plan->spec.tune.elems ? plan->spec.tune.elems : env_int("FFTZ_E", 0);
+        if ((e == 8 || e == 16) && next_pow2((q.L + e - 1) / e) <= 1024) E = e;
     }
     ln.E = E;
     // threads per line
@@ -651,7 +718,7 @@ static void bluestein_tables(TableBuilder& tb, int prec, int64_t n, int64_t M, i
         fb.swap(b);
     } else {
         fb.resize((size_t)M);
-        host_fft_rec(M, b.data(), 1, fb.data());
+        host_fft(M, b.data(), fb.data());
     }
     const double inv = 1.0 / (double)M;
     for (auto& v : fb) v = {v.x * inv, -v.y * inv};
@@ -729,6 +796,47 @@ static bool split_factors(int64_t N, int64_t fmax, std::vector<int64_t>& f)
     return false;
 }
 
+static bool split_ok(int64_t N, const std::vector<int64_t>& f, int64_t fmax)
+{
+    if (f.size() < 2 || f.size() > 3) return false;
+    int64_t prod = 1;
+    for (int64_t v : f) {
+        if (v < 2 || v > fmax || !smooth13(v)) return false;
+        prod *= v;
+    }
+    return prod == N;
+}
+
+std::vector<std::vector<int64_t>> split_candidates(int64_t N, int prec, int maxn)
+{
+    // Ordered 2- and 3-factor splits within one LDS pass per factor, most
+    // balanced first (sum of |log f - log N^(1/p)|), three factors before two.
+    const int64_t fmax = single_pass_limit(prec);
+    std::vector<std::pair<double, std::vector<int64_t>>> c;
+    std::vector<int64_t> divs;
+    for (int64_t d = 16; d <= fmax && d <= N; d++)
+        if (N % d == 0 && smooth13(d)) divs.push_back(d);
+    for (int p = 3; p >= 2; p--) {
+        const double t = std::log((double)N) / p;
+        for (int64_t a : divs)
+            for (int64_t b : divs) {
+                if (N % (a * b)) continue;
+                const int64_t r = N / (a * b);
+                std::vector<int64_t> f = p == 3 ? std::vector<int64_t>{a, b, r} : std::vector<int64_t>{a, N / a};
+                if (p == 2 && b != divs[0]) continue;
+                if (!split_ok(N, f, fmax) || f.back() < 16) continue;
+                double score = (3 - p) * 100.0;
+                for (int64_t v : f) score += std::fabs(std::log((double)v) - t);
+                c.push_back({score, f});
+            }
+    }
+    std::stable_sort(c.begin(), c.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
+    std::vector<std::vector<int64_t>> out;
+    for (const auto& e : c)
+        if ((int)out.size() < maxn && std::find(out.begin(), out.end(), e.second) == out.end()) out.push_back(e.second);
+    return out;
+}
+
 // A length-N transform along one axis of a tensor, through the work buffer,
 // as a four-step over p = 2 or 3 factors (the p = 2 form is
 // core/solvers/complex/pow2_fourstep_solver_dft.c:700-763, with the twiddle
@@ -760,13 +868,17 @@ static bool plan_long_axis(Plan* plan, TableBuilder& tb, const LongAxis& a, std:
     int64_t fmax = env_int(prec == 8 ? "FFTZ_FMAX_D" : "FFTZ_FMAX_F", 256);
     while (fmax < 4096 && fmax * fmax * fmax < N) fmax *= 2;
     std::vector<int64_t> f;
-    if (!split_factors(N, fmax, f)) {
+    const std::vector<int64_t>& tuned = plan->spec.tune.split;
+    const bool use_tuned = split_ok(N, tuned, single_pass_limit(prec));
+    if (use_tuned) {
+        f = tuned;
+    } else if (!split_factors(N, fmax, f)) {
         err = "no four-step factorisation for length " + std::to_string(N);
         return false;
     }
     // power-of-two factors last: the final (transposing) pass then runs on
     // the fixed-length kernel
-    if (!getenv("FFTZ_FACTORS"))
+    if (!use_tuned && !getenv("FFTZ_FACTORS"))
         std::stable_sort(f.begin(), f.end(), [](int64_t x, int64_t y) {
             return ((x & (x - 1)) != 0) > ((y & (y - 1)) != 0);
         });
@@ -1242,20 +1354,55 @@ Plan* batch_plan_create(const std::vector<BatchItem>& items, int prec, int dir, 
         plan->n_transforms += it.batch;
         if (size_index.count(it.n)) continue;
         SizeInfo si;
+        si.n = it.n;
         const int64_t L = smooth13(it.n) ? it.n : bluestein_length(it.n);
         const bool small = next_pow2((L + 7) / 8) <= 256 && (size_t)(lds_pad((int)L) + 2) * esize <= small_budget / 2;
         const bool medium = !small && next_pow2((L + 15) / 16) <= 512 &&
                             (size_t)(lds_pad((int)L) + 2) * esize <= medium_budget;
-        const int cls = small ? 0 : (medium ? 1 : 2);
-        const int cE[3] = {8, 16, 16}, cT[3] = {256, 512, 1024};
-        const size_t cB[3] = {small_budget, medium_budget, big_budget};
-        if (!batch_size_desc(tb, prec, it.n, cE[cls], cT[cls], cB[cls], si, err)) {
-            delete plan;
-            return nullptr;
-        }
-        si.cls = cls;
+        si.cls = small ? 0 : (medium ? 1 : 2);
         size_index[it.n] = (int)sizes.size();
         sizes.push_back(si);
+    }
+    // Per-length descriptors and tables (stage twiddles, Bluestein chirp and
+    // spectrum) are independent: build them on worker threads into separate
+    // arenas, then append each arena to the plan's and rebase its offsets.
+    {
+        const int cE[3] = {8, 16, 16}, cT[3] = {256, 512, 1024};
+        const size_t cB[3] = {small_budget, medium_budget, big_budget};
+        std::vector<std::vector<unsigned char>> arenas(sizes.size());
+        std::vector<std::string> errs(sizes.size());
+        std::vector<char> ok(sizes.size(), 0);
+        std::atomic<size_t> next{0};
+        auto work = [&]() {
+            for (size_t i; (i = next.fetch_add(1)) < sizes.size();) {
+                TableBuilder ltb{arenas[i]};
+                SizeInfo& si = sizes[i];
+                const int cls = si.cls;
+                ok[i] = batch_size_desc(ltb, prec, si.n, cE[cls], cT[cls], cB[cls], si, errs[i]) ? 1 : 0;
+                si.cls = cls;
+            }
+        };
+        const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        const size_t nthr = std::min<size_t>(hw, (sizes.size() + 7) / 8);
+        std::vector<std::thread> pool;
+        for (size_t t = 1; t < nthr; t++) pool.emplace_back(work);
+        work();
+        for (auto& t : pool) t.join();
+        for (size_t i = 0; i < sizes.size(); i++) {
+            if (!ok[i]) {
+                err = errs[i];
+                delete plan;
+                return nullptr;
+            }
+            const int64_t base = tb.add(arenas[i].data(), arenas[i].size());
+            PassDesc& d = sizes[i].d;
+            auto rebase = [base](const void*& ptr) { ptr = (const void*)((intptr_t)ptr + (intptr_t)base); };
+            rebase(d.stw);
+            if (sizes[i].bluestein) {
+                rebase(d.chirp);
+                rebase(d.spec);
+            }
+        }
     }
     plan->in_extent_bytes = in_ext * esize;
     plan->out_extent_bytes = out_ext * esize;
@@ -1437,12 +1584,88 @@ std::string tune_text(int* entries)
 {
     std::lock_guard<std::mutex> g(g_tune_mu);
     std::ostringstream os;
-    os << "# fftz tuned tile shapes: key row_t col_c wg_small wg_medium\n";
-    for (const auto& kv : g_tune)
-        os << kv.first << ' ' << kv.second.row_t << ' ' << kv.second.col_c << ' ' << kv.second.wg_small << ' '
-           << kv.second.wg_medium << '\n';
+    os << "# fftz tuned plans: key row_t col_c wg_small wg_medium [e=elems] [r=L:r.r.r/L:r.r] [f=f.f.f]\n";
+    for (const auto& kv : g_tune) {
+        const TuneParams& t = kv.second;
+        os << kv.first << ' ' << t.row_t << ' ' << t.col_c << ' ' << t.wg_small << ' ' << t.wg_medium;
+        if (t.elems) os << " e=" << t.elems;
+        if (!t.radix_for.empty()) {
+            os << " r=";
+            for (size_t i = 0; i < t.radix_for.size(); i++) {
+                os << (i ? "/" : "") << t.radix_for[i].first << ':';
+                for (size_t j = 0; j < t.radix_for[i].second.size(); j++)
+                    os << (j ? "." : "") << t.radix_for[i].second[j];
+            }
+        }
+        if (!t.split.empty()) {
+            os << " f=";
+            for (size_t i = 0; i < t.split.size(); i++) os << (i ? "." : "") << t.split[i];
+        }
+        os << '\n';
+    }
     if (entries) *entries = (int)g_tune.size();
     return os.str();
+}
+
+// "1.2.3" -> {1, 2, 3}; false on anything but positive integers and dots.
+static bool parse_dotted(const std::string& v, std::vector<int64_t>& out)
+{
+    out.clear();
+    size_t i = 0;
+    while (i < v.size()) {
+        size_t j = v.find('.', i);
+        if (j == std::string::npos) j = v.size();
+        const std::string tok = v.substr(i, j - i);
+        if (tok.empty() || tok.size() > 12 || tok.find_first_not_of("0123456789") != std::string::npos) return false;
+        out.push_back(std::stoll(tok));
+        if (out.back() <= 0) return false;
+        i = j + 1;
+        if (j == v.size()) break;
+        if (i == v.size()) return false;  // trailing dot
+    }
+    return !out.empty();
+}
+
+// Optional fields of a tuned-cache line: e=8, r=1024:16.8.8/256:16.16, f=256.256.256.
+static bool parse_tune_token(const std::string& tok, TuneParams& t)
+{
+    if (tok.size() < 3 || tok[1] != '=') return false;
+    const std::string v = tok.substr(2);
+    std::vector<int64_t> nums;
+    switch (tok[0]) {
+        case 'e':
+            if (v != "8" && v != "16") return false;
+            t.elems = std::stoi(v);
+            return true;
+        case 'f':
+            if (!parse_dotted(v, nums) || nums.size() < 2 || nums.size() > 3) return false;
+            for (int64_t f : nums)
+                if (f < 2) return false;
+            t.split = nums;
+            return true;
+        case 'r': {
+            size_t i = 0;
+            while (i <= v.size()) {
+                size_t j = v.find('/', i);
+                if (j == std::string::npos) j = v.size();
+                const std::string item = v.substr(i, j - i);
+                const size_t c = item.find(':');
+                if (c == std::string::npos || c == 0) return false;
+                const std::string ls = item.substr(0, c);
+                if (ls.size() > 9 || ls.find_first_not_of("0123456789") != std::string::npos) return false;
+                const int L = std::stoi(ls);
+                if (!parse_dotted(item.substr(c + 1), nums)) return false;
+                std::vector<int> r(nums.begin(), nums.end());
+                if (!radix_order_ok(L, r)) return false;
+                t.radix_for.push_back({L, r});
+                i = j + 1;
+                if (j == v.size()) break;
+            }
+            return !t.radix_for.empty();
+        }
+        default:
+            return false;
+    }
 }
 
 int tune_parse(const std::string& text)
@@ -1457,9 +1680,11 @@ int tune_parse(const std::string& text)
         std::istringstream ls(line);
         std::string key, extra;
         TuneParams t;
-        if (!(ls >> key >> t.row_t >> t.col_c >> t.wg_small >> t.wg_medium) || (ls >> extra)) return -1;
+        if (!(ls >> key >> t.row_t >> t.col_c >> t.wg_small >> t.wg_medium)) return -1;
         for (int v : {t.row_t, t.col_c, t.wg_small, t.wg_medium})
             if (v < 0 || v > 4096) return -1;
+        while (ls >> extra)
+            if (!parse_tune_token(extra, t)) return -1;
         got.emplace_back(key, t);
     }
     for (const auto& kv : got) tune_store(kv.first, kv.second);

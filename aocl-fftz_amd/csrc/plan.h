@@ -26,13 +26,26 @@ struct AxisSpec {
 
 enum class FftType : int { C2C = 0, R2C = 1, C2R = 2 };
 
-// Tile-shape choices the autotuner varies; 0 keeps the planner's default.
+// Plan choices the autotuner varies; 0 / empty keeps the planner's default.
 struct TuneParams {
     int row_t = 0;      // threads per row tile (lines per tile = row_t / threads per line)
     int col_c = 0;      // most lines per column tile
     int wg_small = 0;   // uneven batch: workgroups per CU of the small class
     int wg_medium = 0;  // uneven batch: workgroups per CU of the medium class
-    bool empty() const { return !row_t && !col_c && !wg_small && !wg_medium; }
+    int elems = 0;      // complex values per thread per stage (8 or 16)
+    // radix order per pass length: (L, radices in stage order)
+    std::vector<std::pair<int, std::vector<int>>> radix_for;
+    std::vector<int64_t> split;  // four-step factors of the long axis (2 or 3)
+    bool empty() const
+    {
+        return !row_t && !col_c && !wg_small && !wg_medium && !elems && radix_for.empty() && split.empty();
+    }
+    const std::vector<int>* radices(int L) const
+    {
+        for (const auto& r : radix_for)
+            if (r.first == L) return &r.second;
+        return nullptr;
+    }
 };
 
 struct ProblemSpec {
@@ -153,6 +166,13 @@ int tune_import(const char* path);  // entries read, -1 on error or a malformed 
 std::string tune_text(int* entries);           // the cache as text, one problem per line
 int tune_parse(const std::string& text);       // entries stored, -1 (nothing stored) if malformed
 void tune_clear();
+
+// Autotuner search spaces: radix orders of a pass length (the planner's
+// default first, then other multisets of supported radices with at most one
+// more stage, in descending and ascending order), and ordered 2- / 3-factor
+// four-step splits of a long axis, most balanced first; at most maxn each.
+std::vector<std::vector<int>> radix_orders(int L, int maxn);
+std::vector<std::vector<int64_t>> split_candidates(int64_t N, int prec_bytes, int maxn);
 
 // Kernel time of `reps` executes of p on (in, out) (device buffers) on a slot
 // stream, in ms; negative on a HIP error.

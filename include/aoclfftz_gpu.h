@@ -71,18 +71,25 @@ aoclfftz_error_type aoclfftz_gpu_set_graph(FFTZ_VOID *handle, FFTZ_INT32 enable)
  * of the plan qualifies.  Not to be called while the handle executes. */
 aoclfftz_error_type aoclfftz_gpu_set_compute_mode(FFTZ_VOID *handle, FFTZ_INT32 mode);
 
-/* Autotuner.  Times the handle's candidate tile shapes on the given device
- * buffers (reps executes each; in / out are overwritten; an in-place handle
- * takes in == out) and keeps the
- * fastest: row-tile width and column-tile lines for an ordinary plan,
- * workgroups per CU of each tile class for an uneven batch.  The choice is
- * also stored in a process-wide cache keyed by the problem, which later
- * setups of the same problem use without tuning again.  No counterpart in
- * the reference (its plan choice is a cost model, selector/selector.c). */
+/* Autotuner.  Times candidate plans of the handle's problem on the given
+ * device buffers (reps executes each; in / out are overwritten; an in-place
+ * handle takes in == out) and keeps the fastest.  For an ordinary plan the
+ * search runs in steps, each starting from the best so far: row-tile width x
+ * column-tile lines; the four-step split of a long 1-D axis (2 or 3 factors);
+ * the radix order of each pass length (other multisets of the supported
+ * radices, descending and ascending); complex values per thread (8 or 16).
+ * For an uneven batch it times workgroups per CU of each tile class.  The
+ * choice is also stored in a process-wide cache keyed by the problem, which
+ * later setups of the same problem use without tuning again.  No counterpart
+ * in the reference, whose plan choice is a cost model over the same kinds of
+ * decision (selector/selector.c; radix plan core/common/pow2_radix_decompose.c:122-198,
+ * radix search selector/complex/selector_ct_dft.c:104-328). */
 aoclfftz_error_type aoclfftz_gpu_autotune(FFTZ_VOID *handle, FFTZ_VOID *in, FFTZ_VOID *out, FFTZ_INT32 reps);
 
 /* Write / read the tuned cache as text (one problem per line:
- * "key row_t col_c wg_small wg_medium"; '#' starts a comment line).  Return
+ * "key row_t col_c wg_small wg_medium [e=E] [r=L:r.r.r/L:r.r] [f=f.f.f]",
+ * the optional fields being complex values per thread, radix orders per pass
+ * length and the four-step split; '#' starts a comment line).  Return
  * the number of entries, or -1 when the file cannot be opened or a line is
  * malformed (then nothing is imported).  The counterpart of FFTW wisdom files
  * for this library; the reference's wisdom calls are stubs
@@ -184,6 +191,19 @@ FFTZ_VOID *aoclfftz_gpu_server_create(FFTZ_INT64 hbm_budget_bytes, FFTZ_INT32 ma
  * request id (>= 0), -1 for an invalid problem, -2 when the footprint alone
  * exceeds the budget.  Thread-safe; returns without waiting. */
 FFTZ_INT64 aoclfftz_gpu_server_submit(FFTZ_VOID *server, FFTZ_VOID *problem, FFTZ_INT32 precision_bytes);
+
+/* Open-loop load generator: submits problems[i] (as aoclfftz_gpu_server_submit,
+ * precision_bytes[i] 4 or 8) at arrival_s[i] seconds after the call starts,
+ * spinning on the host clock in between, and writes the request ids (or the
+ * submit error codes) to ids[i].  Each request's latency is measured from its
+ * scheduled arrival, so a generator that falls behind shows up as latency,
+ * not as a lower offered rate; *max_late_s (optional) receives the largest
+ * delay of a submission behind its schedule.  Returns count, or -1 on bad
+ * arguments (arrival times must not decrease).  Blocks the calling thread
+ * until the last arrival.  No counterpart in the reference. */
+FFTZ_INT64 aoclfftz_gpu_server_submit_timed(FFTZ_VOID *server, FFTZ_VOID **problems,
+                                           const FFTZ_INT32 *precision_bytes, const FFTZ_DOUBLE *arrival_s,
+                                           FFTZ_INT64 count, FFTZ_INT64 *ids, FFTZ_DOUBLE *max_late_s);
 
 /* 1 when request id has completed (or failed), 0 while queued or running,
  * -1 for an unknown id. */

@@ -120,3 +120,66 @@ def test_fftw_wisdom_file_forms(tmp_path):
     assert lib.fftw_import_wisdom_from_file(None) == 0
     lib.fftw_forget_wisdom()
     assert lines_per_tile() == 4
+
+
+KEY24 = "p8:0:0:1:0|16777216,1,1;|1,16777216,16777216;"
+
+
+def test_extended_fields_choose_factorisation():
+    """The optional fields of a tuned-cache line pick the radix order of a
+    pass length (r=), complex values per thread (e=) and the four-step split
+    (f=); a dry-run plan of the problem shows them, and export writes them
+    back unchanged."""
+    fftz.tune_clear()
+    pj = fftz.plan_json([(1024, 1, 1)], [(4096, 1024, 1024)], "f", 0)
+    assert pj["launches"][0]["radix"] == [16, 16, 4] and pj["launches"][0]["fixed"]
+    assert fftz.tune_parse(f"{KEY} 0 0 0 0 r=1024:8.8.16 e=8\n") == 1
+    ln = fftz.plan_json([(1024, 1, 1)], [(4096, 1024, 1024)], "f", 0)["launches"][0]
+    assert ln["radix"] == [8, 8, 16] and not ln["fixed"] and ln["E"] == 8
+    assert f"{KEY} 0 0 0 0 e=8 r=1024:8.8.16" in fftz.tune_text()
+    n = 1 << 24
+    dflt = fftz.plan_json([(n, 1, 1)], [(1, n, n)], "d", 0, inplace=True)
+    assert [l["L"] for l in dflt["launches"]] == [256, 256, 256]
+    assert fftz.tune_parse(f"{KEY24} 0 0 0 0 f=128.256.512 r=256:4.4.16\n") == 1
+    pj = fftz.plan_json([(n, 1, 1)], [(1, n, n)], "d", 0, inplace=True)
+    assert [l["L"] for l in pj["launches"]] == [128, 256, 512]
+    assert pj["launches"][1]["radix"] == [4, 4, 16]
+    assert f"{KEY24} 0 0 0 0 r=256:4.4.16 f=128.256.512" in fftz.tune_text()
+    fftz.tune_clear()
+
+
+def test_extended_fields_malformed():
+    fftz.tune_clear()
+    for bad in ("r=1000:8.8.16", "r=1024:8.8.16.", "r=1024:32.32", "r=:16", "e=12", "f=1.2",
+                "f=16", "x=3", "r=1024", "f=4096.4096.4096.4096"):
+        assert fftz.tune_parse(f"{KEY} 0 0 0 0 {bad}\n") == -1, bad
+    assert fftz.tune_text().count("\n") == 1
+    # a split whose product is not the axis length is ignored by the planner
+    assert fftz.tune_parse(f"{KEY24} 0 0 0 0 f=256.256.128\n") == 1
+    n = 1 << 24
+    pj = fftz.plan_json([(n, 1, 1)], [(1, n, n)], "d", 0, inplace=True)
+    assert [l["L"] for l in pj["launches"]] == [256, 256, 256]
+    fftz.tune_clear()
+
+
+def test_tuned_split_geometry_in_emulator():
+    """A tuned four-step split and radix orders replayed by the CPU plan
+    emulator (bounds-checked gathers / scatters, FP64 line transforms) give
+    the DFT of the input."""
+    import numpy as np
+    import plan_emulator as E
+    n = 1 << 16
+    key = f"p8:0:0:0:0|{n},1,1;|2,{n},{n};"
+    fftz.tune_clear()
+    assert fftz.tune_parse(f"{key} 0 0 0 0 f=16.64.64 r=64:4.16\n") == 1
+    pj = fftz.plan_json([(n, 1, 1)], [(2, n, n)], "d", 0)
+    assert [l["L"] for l in pj["launches"]] == [16, 64, 64]
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal(2 * n) + 1j * rng.standard_normal(2 * n)
+    y = np.zeros_like(x)
+    E.check_launch_limits(pj)
+    work = np.zeros(pj["work_bytes"] // 16, np.complex128)
+    E.run(pj, {0: x.copy(), 1: y, 2: work})
+    ref = np.fft.fft(x.reshape(2, n)).reshape(-1)
+    assert np.linalg.norm(y - ref) / np.linalg.norm(ref) < 1e-13
+    fftz.tune_clear()
