@@ -233,10 +233,14 @@ FFTZ_VOID* aoclfftz_gpu_server_create(FFTZ_INT64 hbm_budget_bytes, FFTZ_INT32 ma
     return s;
 }
 
-FFTZ_INT64 aoclfftz_gpu_server_submit(FFTZ_VOID* server, FFTZ_VOID* problem, FFTZ_INT32 precision_bytes)
+}  // extern "C"
+
+namespace fftz {
+namespace {
+
+// Queue one request whose arrival time (the start of its latency) is t_arrive.
+int64_t submit_at(Server* s, void* problem, int precision_bytes, double t_arrive)
 {
-    Server* s = fftz::as_server(server);
-    if (!s) return -1;
     fftz::ProblemSpec spec;
     const void* in = nullptr;
     void* out = nullptr;
@@ -282,13 +286,49 @@ FFTZ_INT64 aoclfftz_gpu_server_submit(FFTZ_VOID* server, FFTZ_VOID* problem, FFT
     }
     std::lock_guard<std::mutex> g(s->mu);
     r->id = s->next_id++;
-    r->t_submit = fftz::now_s();
+    r->t_submit = t_arrive;
     if (s->t_first < 0) s->t_first = r->t_submit;
     s->reqs[r->id] = r;
     s->queue.push_back(r);
     s->queue_peak = std::max(s->queue_peak, s->queue.size());
     s->cv_work.notify_one();
     return r->id;
+}
+
+}  // namespace
+}  // namespace fftz
+
+extern "C" {
+
+FFTZ_INT64 aoclfftz_gpu_server_submit(FFTZ_VOID* server, FFTZ_VOID* problem, FFTZ_INT32 precision_bytes)
+{
+    Server* s = fftz::as_server(server);
+    if (!s) return -1;
+    return fftz::submit_at(s, problem, precision_bytes, fftz::now_s());
+}
+
+FFTZ_INT64 aoclfftz_gpu_server_submit_timed(FFTZ_VOID* server, FFTZ_VOID** problems,
+                                           const FFTZ_INT32* precision_bytes, const FFTZ_DOUBLE* arrival_s,
+                                           FFTZ_INT64 count, FFTZ_INT64* ids, FFTZ_DOUBLE* max_late_s)
+{
+    Server* s = fftz::as_server(server);
+    if (!s || count < 0 || (count > 0 && (!problems || !precision_bytes || !arrival_s || !ids))) return -1;
+    for (int64_t i = 1; i < count; i++)
+        if (arrival_s[i] < arrival_s[i - 1]) {
+            fftz::report_msg("aoclfftz_gpu_server_submit_timed", "arrival times must not decrease");
+            return -1;
+        }
+    const double t0 = fftz::now_s();
+    double late = 0.0;
+    for (int64_t i = 0; i < count; i++) {
+        const double due = t0 + arrival_s[i];
+        double t = fftz::now_s();
+        while (t < due) t = fftz::now_s();  // open loop: spin to the arrival time
+        late = std::max(late, t - due);
+        ids[i] = fftz::submit_at(s, problems[i], precision_bytes[i], due);
+    }
+    if (max_late_s) *max_late_s = late;
+    return count;
 }
 
 FFTZ_INT32 aoclfftz_gpu_server_test(FFTZ_VOID* server, FFTZ_INT64 id)

@@ -75,7 +75,7 @@ EXPORTS = (
     "aoclfftz_gpu_plan_info", "aoclfftz_gpu_plan_flops", "aoclfftz_gpu_plan_device_bytes",
     "aoclfftz_gpu_plan_json", "aoclfftz_gpu_profile", "aoclfftz_gpu_profile_read",
     "aoclfftz_gpu_plan_launches", "aoclfftz_gpu_execute_many", "aoclfftz_gpu_batch_setup",
-    "aoclfftz_gpu_batch_plan_json", "aoclfftz_gpu_execute_noalloc",
+    "aoclfftz_gpu_batch_plan_json", "aoclfftz_gpu_execute_noalloc", "aoclfftz_gpu_server_submit_timed",
 )
 
 
@@ -133,6 +133,10 @@ def load():
     lib.aoclfftz_gpu_server_create.restype = vp
     lib.aoclfftz_gpu_server_submit.argtypes = [vp, vp, ctypes.c_int32]
     lib.aoclfftz_gpu_server_submit.restype = ctypes.c_int64
+    lib.aoclfftz_gpu_server_submit_timed.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(ctypes.c_int32),
+                                                     ctypes.POINTER(ctypes.c_double), ctypes.c_int64,
+                                                     ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double)]
+    lib.aoclfftz_gpu_server_submit_timed.restype = ctypes.c_int64
     lib.aoclfftz_gpu_server_test.argtypes = [vp, ctypes.c_int64]
     lib.aoclfftz_gpu_server_test.restype = ctypes.c_int32
     lib.aoclfftz_gpu_server_wait.argtypes = [vp, ctypes.c_int64, ctypes.POINTER(ctypes.c_double)]
@@ -343,6 +347,28 @@ class Server:
         d = make_desc(dims, vecs, inp=inp, out=inp if inplace else out, fft_type=fft_type, direction=direction,
                       placement=0 if inplace else 1, ilp64=True)
         return load().aoclfftz_gpu_server_submit(self.handle, ctypes.byref(d), 4 if precision == "f" else 8)
+
+    def submit_timed(self, requests, arrival_s):
+        """Open-loop submission from native code: requests[i] is a dict of
+        submit() keyword arguments (dims, vecs, precision, direction, fft_type,
+        inp, out, inplace), submitted arrival_s[i] seconds after the call
+        starts.  Returns (ids, largest lateness behind the schedule in s)."""
+        n = len(requests)
+        descs = []
+        for r in requests:
+            inplace = r.get("inplace", False)
+            descs.append(make_desc(r["dims"], r["vecs"], inp=r["inp"], out=r["inp"] if inplace else r["out"],
+                                   fft_type=r.get("fft_type", 0), direction=r.get("direction", 0),
+                                   placement=0 if inplace else 1, ilp64=True))
+        probs = (ctypes.c_void_p * max(1, n))(*[ctypes.addressof(d) for d in descs])
+        precs = (ctypes.c_int32 * max(1, n))(*[4 if r.get("precision", "f") == "f" else 8 for r in requests])
+        arr = (ctypes.c_double * max(1, n))(*[float(t) for t in arrival_s])
+        ids = (ctypes.c_int64 * max(1, n))()
+        late = ctypes.c_double(0.0)
+        rc = load().aoclfftz_gpu_server_submit_timed(self.handle, probs, precs, arr, n, ids, ctypes.byref(late))
+        if rc < 0:
+            raise ValueError("aoclfftz_gpu_server_submit_timed rejected its arguments")
+        return list(ids[:n]), late.value
 
     def test(self, rid: int) -> int:
         return load().aoclfftz_gpu_server_test(self.handle, rid)

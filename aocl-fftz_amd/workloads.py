@@ -74,29 +74,50 @@ def sinusoids(n: int, batch: int, seed: int = 4, tones: int = 8, noise: float = 
     return out
 
 
+def _size_menu(rng, lo: int, hi: int, count: int, prime_frac: float = 0.0):
+    """count distinct lengths in [lo, hi], log-uniform: 7-smooth lengths, plus
+    a prime_frac share of primes (Bluestein), from rng."""
+    sm = [v for v in smooth7(lo, hi)]
+    pr = [v for v in primes(lo, hi)] if prime_frac > 0 else []
+    out = set()
+    while len(out) < count:
+        target = float(np.exp(rng.uniform(np.log(lo), np.log(hi))))
+        pool = pr if (pr and rng.random() < prime_frac) else sm
+        i = int(np.searchsorted(pool, target))
+        out.add(int(pool[min(i, len(pool) - 1)]))
+    return sorted(out)
+
+
 def serving_requests(count: int = 4000, seed: int = 11, max_elems: int = 1 << 21):
-    """Request mix for the serving loop: 50% 1-D C2C (N = 2^6 .. 2^20), 25% 2-D
-    R2C (128^2 .. 4096^2), 25% 3-D C2C (32^3 .. 256^3); 10% complex128; batch
-    a power of two in 1..256, reduced so one request holds at most max_elems
-    elements.  Each entry: (dims, vecs, precision, fft_type, in_elems,
-    out_elems, in_real)."""
+    """Request mix for the serving loop: 50% 1-D C2C (N in [64, 2^20]), 25% 2-D
+    R2C (N x N, N in [128, 4096]), 25% 3-D C2C (N^3, N in [32, 256]); 10%
+    complex128; batch in 1..256, reduced so one request holds at most max_elems
+    elements.  Lengths come from seeded menus of log-uniform 7-smooth lengths
+    (the 1-D menu also holds primes, which run Bluestein), batches from a
+    seeded menu of integers, so the number of distinct problems (resident
+    plans) stays bounded.  Each entry: (dims, vecs, precision, fft_type,
+    in_elems, out_elems, in_real)."""
     rng = np.random.default_rng(seed)
+    menu1 = _size_menu(rng, 64, 1 << 20, 24, prime_frac=0.2)
+    menu2 = _size_menu(rng, 128, 4096, 10)
+    menu3 = _size_menu(rng, 32, 256, 8)
+    batches = sorted(set(int(v) for v in np.round(np.exp(rng.uniform(0, np.log(256), 12)))) | {1, 256})
     out = []
     for _ in range(count):
         kind = int(rng.choice(3, p=[0.5, 0.25, 0.25]))
         prec = "d" if rng.random() < 0.1 else "f"
-        b = 1 << int(rng.integers(0, 9))
+        b = int(rng.choice(batches))
         if kind == 0:
-            n = 1 << int(rng.integers(6, 21))
+            n = int(rng.choice(menu1))
             b = max(1, min(b, max_elems // n))
             out.append(([(n, 1, 1)], [(b, n, n)], prec, 0, n * b, n * b, False))
         elif kind == 1:
-            n = 1 << int(rng.integers(7, 13))
+            n = int(rng.choice(menu2))
             h = n // 2 + 1
             b = max(1, min(b, max_elems // (n * n)))
             out.append(([(n, 1, 1), (n, n, h)], [(b, n * n, n * h)], prec, 1, n * n * b, n * h * b, True))
         else:
-            n = 1 << int(rng.integers(5, 9))
+            n = int(rng.choice(menu3))
             b = max(1, min(b, max_elems // n ** 3))
             out.append(([(n, 1, 1), (n, n, n), (n, n * n, n * n)], [(b, n ** 3, n ** 3)], prec, 0, n ** 3 * b,
                         n ** 3 * b, False))

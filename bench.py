@@ -603,9 +603,11 @@ def bench_split(torch, fftz, steps, warmup, n=4096, b=16384):
 
 def bench_serving(torch, fftz, rates=(2000, 10000, 50000), seconds=0.4, max_count=4000, inflight=16):
     """Serving: open-loop Poisson arrivals of the serving request mix (1-D C2C,
-    2-D R2C, 3-D C2C, 10% complex128) through one server (16 slots, 16 GiB
-    budget); request buffers are carved from a ring in one device arena.
-    Plans are warmed first; p50 / p99 latency and throughput per rate."""
+    2-D R2C, 3-D C2C of seeded non-power-of-two sizes, 10% complex128) through
+    one server (16 slots, 16 GiB budget); request buffers are carved from a
+    ring in one device arena.  Plans are warmed first; requests are submitted
+    at their arrival times by the library's native generator and latency runs
+    from the arrival time; p50 / p99 latency and throughput per rate."""
     import numpy as np
     import workloads
     reqs = workloads.serving_requests(count=max_count)
@@ -623,12 +625,17 @@ def bench_serving(torch, fftz, rates=(2000, 10000, 50000), seconds=0.4, max_coun
         ring[0] += nbytes
         return base + off
 
-    def submit(r):
+    def request(r):
         dims, vecs, prec, ftype, nin, nout, in_real = r
         esz = 8 if prec == "f" else 16
         inp = carve(nin * (esz // 2 if in_real else esz))
         out = carve(nout * esz)
-        return srv.submit(dims, vecs, prec, 0, fft_type=ftype, inp=inp, out=out)
+        return {"dims": dims, "vecs": vecs, "precision": prec, "fft_type": ftype, "inp": inp, "out": out}
+
+    def submit(r):
+        q = request(r)
+        return srv.submit(q["dims"], q["vecs"], q["precision"], 0, fft_type=q["fft_type"], inp=q["inp"],
+                          out=q["out"])
 
     seen = set()
     for r in reqs:  # warm: one request per distinct problem builds its plan
@@ -644,21 +651,19 @@ def bench_serving(torch, fftz, rates=(2000, 10000, 50000), seconds=0.4, max_coun
     for rate in rates:
         count = int(min(max_count, rate * seconds))
         gaps = rng.exponential(1.0 / rate, count)
+        arrivals = np.cumsum(gaps)
+        batch = [request(reqs[i]) for i in range(count)]
         srv.reset_stats()
-        ids = []
-        t0 = time.perf_counter()
-        t_next = t0
-        for i in range(count):
-            t_next += gaps[i]
-            while time.perf_counter() < t_next:
-                pass
-            ids.append(submit(reqs[i]))
-        t_sub = time.perf_counter() - t0
+        # open loop from native code (aoclfftz_gpu_server_submit_timed): each
+        # request is submitted at its arrival time and its latency counts from
+        # that time
+        ids, late = srv.submit_timed(batch, arrivals)
         for rid in ids:
             srv.wait(rid)
         st = srv.stats()
         out[f"rate_{rate}"] = {"requests": count, "offered_per_s": rate,
-                               "submitted_per_s": round(count / t_sub, 1),
+                               "submitted_per_s": round(count / float(arrivals[-1]), 1),
+                               "max_submit_late_ms": round(late * 1e3, 4),
                                "p50_ms": round(st["p50_ms"], 4), "p99_ms": round(st["p99_ms"], 4),
                                "max_ms": round(st["max_ms"], 4),
                                "transforms_per_s": round(st["transforms_per_s"], 1),
