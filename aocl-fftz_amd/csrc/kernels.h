@@ -28,8 +28,9 @@ const void* kernel_symbol_batch_f32(int E, int maxt);
 const void* kernel_symbol_batch_f64(int E, int maxt);
 
 // Fixed-length pass kernel for power-of-two L in [64, 4096] (radix plan 16s
-// then remainder, 16 elements per thread, block <= 256); nullptr otherwise.
-const void* kernel_symbol_fixed(int prec_bytes, int L);
+// then remainder, 16 elements per thread), block <= maxt (256, or 512 for
+// L >= 256); nullptr otherwise.
+const void* kernel_symbol_fixed(int prec_bytes, int L, int maxt);
 
 // Matrix-core pass (kernels_mfma.hip) for complex64, L = 256 or 4096, with
 // operand precision `mode` (MfmaMode) and nb (2 or 4) column blocks per

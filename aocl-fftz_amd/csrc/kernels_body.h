@@ -72,6 +72,24 @@ __device__ __forceinline__ LineIdx line_index(const PassDesc& p, const TileIo& i
     return li;
 }
 
+// Line held by tile slot c (l_base = tile index x C): line l_base + c for
+// flat tiles; for ragged column tiles the slot's row and position in the row
+// (PassDesc::vrow), live only below the row length.
+struct SlotLine {
+    uint32_t l;
+    bool live;
+};
+
+__device__ __forceinline__ SlotLine slot_line(const PassDesc& p, const TileIo& io, int64_t l_base, int c)
+{
+    const int64_t v = l_base + c;
+    if (p.vrow.d == 0) return SlotLine{(uint32_t)v, v < io.nlines};
+    const uint32_t row = fdiv((uint32_t)v, p.vrow);
+    const uint32_t cc = (uint32_t)v - row * p.vrow.d;
+    const uint32_t l = row * p.ln0.d + cc;
+    return SlotLine{l, cc < p.ln0.d && (int64_t)l < io.nlines};
+}
+
 template <typename R>
 __device__ __forceinline__ cplx<R> to_prec(double x, double y) { return {R(x), R(y)}; }
 
@@ -97,9 +115,9 @@ __device__ __forceinline__ void load_tile(const PassDesc& p, const TileIo& io, c
     const bool col = (p.flags & kInCol) != 0;
     const int c = col ? (tid & (p.C - 1)) : (tid >> p.tl_shift);
     const int t = col ? (tid >> p.c_shift) : (tid & (p.TL - 1));
-    const int64_t l = l_base + c;
-    if (l >= io.nlines) return;
-    LineIdx li = line_index(p, io, (uint32_t)l);
+    const SlotLine sl = slot_line(p, io, l_base, c);
+    if (!sl.live) return;
+    LineIdx li = line_index(p, io, sl.l);
     cplx<R>* line = lds + c * p.LP;
     if (p.flags & kRealIn) {
         // R2C: element j of the L-point complex line is (x[2j], x[2j+1])
@@ -283,9 +301,9 @@ __device__ __forceinline__ void r2c_recombine_store(const PassDesc& p, const Til
 {
     const int c = tid >> p.tl_shift;
     const int t = tid & (p.TL - 1);
-    const int64_t l = l_base + c;
-    if (l >= io.nlines) return;
-    LineIdx li = line_index(p, io, (uint32_t)l);
+    const SlotLine sl = slot_line(p, io, l_base, c);
+    if (!sl.live) return;
+    LineIdx li = line_index(p, io, sl.l);
     const cplx<R>* line = lds + c * p.LP;
     cplx<R>* dst = reinterpret_cast<cplx<R>*>(io.out) + li.ooff;
     const cplx<R>* rtw = reinterpret_cast<const cplx<R>*>(p.rtw);
@@ -322,9 +340,9 @@ __device__ __forceinline__ void store_tile(const PassDesc& p, const TileIo& io, 
     const bool col = (p.flags & kOutCol) != 0;
     const int c = col ? (tid & (p.C - 1)) : (tid >> p.tl_shift);
     const int t = col ? (tid >> p.c_shift) : (tid & (p.TL - 1));
-    const int64_t l = l_base + c;
-    if (l >= io.nlines) return;
-    LineIdx li = line_index(p, io, (uint32_t)l);
+    const SlotLine sl = slot_line(p, io, l_base, c);
+    if (!sl.live) return;
+    LineIdx li = line_index(p, io, sl.l);
     const cplx<R>* line = lds + c * p.LP;
     cplx<R> v[E];
 #pragma unroll
@@ -631,12 +649,14 @@ __device__ __forceinline__ void pass_body_direct(const PassDesc& p, const TileIo
     const int ti = in_col ? (tid >> p.c_shift) : (tid & (p.TL - 1));
     const int co = out_col ? (tid & (p.C - 1)) : (tid >> p.tl_shift);
     const int to = out_col ? (tid >> p.c_shift) : (tid & (p.TL - 1));
-    const bool live_in = ci < p.C && (l_base + ci) < io.nlines;
-    const bool live_out = co < p.C && (l_base + co) < io.nlines;
+    const SlotLine sli = slot_line(p, io, l_base, ci);
+    const SlotLine slo = slot_line(p, io, l_base, co);
+    const bool live_in = ci < p.C && sli.live;
+    const bool live_out = co < p.C && slo.live;
     const cplx<R>* src = nullptr;
     int64_t gin = 0;
     if (live_in) {
-        LineIdx li = line_index(p, io, (uint32_t)(l_base + ci));
+        LineIdx li = line_index(p, io, sli.l);
         src = reinterpret_cast<const cplx<R>*>(io.in) + li.ioff;
         gin = li.gin;
     }
@@ -644,7 +664,7 @@ __device__ __forceinline__ void pass_body_direct(const PassDesc& p, const TileIo
     uint32_t l0 = 0;
     int64_t gout = 0;
     if (live_out) {
-        LineIdx li = line_index(p, io, (uint32_t)(l_base + co));
+        LineIdx li = line_index(p, io, slo.l);
         dst = reinterpret_cast<cplx<R>*>(io.out) + li.ooff;
         l0 = li.l0;
         gout = li.gout;
@@ -656,7 +676,7 @@ __device__ __forceinline__ void pass_body_direct(const PassDesc& p, const TileIo
         uint32_t l01 = 0;
         int64_t gout1 = 0;
         if (live_in) {
-            LineIdx li = line_index(p, io, (uint32_t)(l_base + ci));
+            LineIdx li = line_index(p, io, sli.l);
             d1 = reinterpret_cast<cplx<R>*>(io.out) + li.ooff;
             l01 = li.l0;
             gout1 = li.gout;
@@ -723,10 +743,10 @@ __device__ __forceinline__ void bluestein_body(const PassDesc& p, const TileIo& 
 {
     const int c = tid >> p.tl_shift;
     const int t = tid & (p.TL - 1);
-    const int64_t l = l_base + c;
-    const bool live = l < io.nlines;
+    const SlotLine sl = slot_line(p, io, l_base, c);
+    const bool live = sl.live;
     LineIdx li = {0, 0, 0};
-    if (live) li = line_index(p, io, (uint32_t)l);
+    if (live) li = line_index(p, io, sl.l);
     cplx<R>* line = lds + c * p.LP;
     const cplx<R>* chirp = reinterpret_cast<const cplx<R>*>(p.chirp);
     const cplx<R>* spec = reinterpret_cast<const cplx<R>*>(p.spec);
@@ -907,11 +927,12 @@ __device__ __forceinline__ void bluestein_body_direct(const PassDesc& p, const T
 {
     const int c = tid >> p.tl_shift;
     const int t = tid & (p.TL - 1);
-    const bool live = c < p.C && (l_base + c) < io.nlines;
+    const SlotLine sl = slot_line(p, io, l_base, c);
+    const bool live = c < p.C && sl.live;
     const cplx<R>* src = nullptr;
     cplx<R>* dst = nullptr;
     if (live) {
-        LineIdx li = line_index(p, io, (uint32_t)(l_base + c));
+        LineIdx li = line_index(p, io, sl.l);
         // real-side lines are offset in real elements (and then only read /
         // written through real pointers)
         src = (p.flags & kRealPromote)

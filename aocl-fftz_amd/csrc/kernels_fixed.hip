@@ -45,6 +45,21 @@ __device__ __forceinline__ LineOff line_off(const PassDesc& p, uint32_t l)
     return o;
 }
 
+// Line of tile slot c and whether it exists (flat tiles: l_base + c; ragged
+// column tiles: PassDesc::vrow).
+__device__ __forceinline__ bool slot_line_f(const PassDesc& p, int64_t l_base, int c, uint32_t& l)
+{
+    const int64_t v = l_base + c;
+    if (p.vrow.d == 0) {
+        l = (uint32_t)v;
+        return v < p.nlines;
+    }
+    const uint32_t row = fdiv_u((uint32_t)v, p.vrow);
+    const uint32_t cc = (uint32_t)v - row * p.vrow.d;
+    l = row * p.ln0.d + cc;
+    return cc < p.ln0.d && (int64_t)l < p.nlines;
+}
+
 // Radix plan of a power-of-two L: 16s first, remainder last.
 template <int L>
 struct Plan2 {
@@ -125,8 +140,9 @@ template <typename R, int L>
 __device__ __forceinline__ void fixed_load(const PassDesc& p, int64_t l_base, int c, int t, cplx<R>* v)
 {
     constexpr int TL = L / 16;
-    if ((l_base + c) < p.nlines) {
-        LineOff lo = line_off(p, (uint32_t)(l_base + c));
+    uint32_t l;
+    if (slot_line_f(p, l_base, c, l)) {
+        LineOff lo = line_off(p, l);
         const cplx<R>* src = reinterpret_cast<const cplx<R>*>(p.in) + lo.ioff + (int64_t)t * p.ies;
         const int64_t step = (int64_t)TL * p.ies;
 #pragma unroll
@@ -137,11 +153,12 @@ __device__ __forceinline__ void fixed_load(const PassDesc& p, int64_t l_base, in
     }
 }
 
-// One workgroup per tile of C lines (grid = number of tiles).  A register
-// prefetch of the next tile was measured slower: the extra 16 live values per
-// lane cut occupancy from 5 to 2 waves/SIMD.
-template <typename R, int L>
-__global__ void __launch_bounds__(256) k_fixed_pass(const PassDesc p)
+// One workgroup per tile of C lines (grid = number of tiles), at most MAXT
+// threads (256, or 512 for wide column tiles).  A register prefetch of the
+// next tile was measured slower: the extra 16 live values per lane cut
+// occupancy from 5 to 2 waves/SIMD.
+template <typename R, int L, int MAXT>
+__global__ void __launch_bounds__(MAXT) k_fixed_pass(const PassDesc p)
 {
     using P = Plan2<L>;
     constexpr int S = P::S;
@@ -198,8 +215,9 @@ __global__ void __launch_bounds__(256) k_fixed_pass(const PassDesc p)
             for (int i = 0; i < RL; i++) w[k][i] = line[pad_c<L>(i * lsL + j)];
         }
     }
-    if ((l_base + co) >= p.nlines) return;
-    LineOff lo = line_off(p, (uint32_t)(l_base + co));
+    uint32_t l_out;
+    if (!slot_line_f(p, l_base, co, l_out)) return;
+    LineOff lo = line_off(p, l_out);
     const cplx<R>* twl = reinterpret_cast<const cplx<R>*>(p.stw) + p.twoff[S - 1];
     cplx<R>* dst = reinterpret_cast<cplx<R>*>(p.out) + lo.ooff;
     const cplx<double>* t4 = reinterpret_cast<const cplx<double>*>(p.tw4);
@@ -237,30 +255,26 @@ __global__ void __launch_bounds__(256) k_fixed_pass(const PassDesc p)
     }
 }
 
-const void* kernel_symbol_fixed(int prec_bytes, int L)
+template <typename R, int MAXT>
+static const void* fixed_sym(int L)
 {
-    if (prec_bytes == 4) {
-        switch (L) {
-            case 64: return (const void*)&k_fixed_pass<float, 64>;
-            case 128: return (const void*)&k_fixed_pass<float, 128>;
-            case 256: return (const void*)&k_fixed_pass<float, 256>;
-            case 512: return (const void*)&k_fixed_pass<float, 512>;
-            case 1024: return (const void*)&k_fixed_pass<float, 1024>;
-            case 2048: return (const void*)&k_fixed_pass<float, 2048>;
-            case 4096: return (const void*)&k_fixed_pass<float, 4096>;
-            default: return nullptr;
-        }
-    }
     switch (L) {
-        case 64: return (const void*)&k_fixed_pass<double, 64>;
-        case 128: return (const void*)&k_fixed_pass<double, 128>;
-        case 256: return (const void*)&k_fixed_pass<double, 256>;
-        case 512: return (const void*)&k_fixed_pass<double, 512>;
-        case 1024: return (const void*)&k_fixed_pass<double, 1024>;
-        case 2048: return (const void*)&k_fixed_pass<double, 2048>;
-        case 4096: return (const void*)&k_fixed_pass<double, 4096>;
+        case 64: return MAXT == 256 ? (const void*)&k_fixed_pass<R, 64, 256> : nullptr;
+        case 128: return MAXT == 256 ? (const void*)&k_fixed_pass<R, 128, 256> : nullptr;
+        case 256: return (const void*)&k_fixed_pass<R, 256, MAXT>;
+        case 512: return (const void*)&k_fixed_pass<R, 512, MAXT>;
+        case 1024: return (const void*)&k_fixed_pass<R, 1024, MAXT>;
+        case 2048: return (const void*)&k_fixed_pass<R, 2048, MAXT>;
+        case 4096: return (const void*)&k_fixed_pass<R, 4096, MAXT>;
         default: return nullptr;
     }
+}
+
+const void* kernel_symbol_fixed(int prec_bytes, int L, int maxt)
+{
+    if (maxt != 256 && maxt != 512) return nullptr;
+    if (prec_bytes == 4) return maxt == 256 ? fixed_sym<float, 256>(L) : fixed_sym<float, 512>(L);
+    return maxt == 256 ? fixed_sym<double, 256>(L) : fixed_sym<double, 512>(L);
 }
 
 }  // namespace fftz

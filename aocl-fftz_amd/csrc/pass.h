@@ -98,6 +98,11 @@ struct PassDesc {
     int64_t nchirp;        // Bluestein: n (valid input / output elements)
     int64_t tw4_n;         // four-step twiddle base N
     FastDiv ln0, ln1;      // line level counts l0, l1 (l2 = rest)
+    // Ragged column tiles (a row of ln0 lines that C does not divide): the
+    // tile index times C is a virtual line index v; row = v / vrow.d with
+    // vrow.d = tiles per row x C, and slot v % vrow.d is line l0 of that row
+    // when below ln0 (idle otherwise).  vrow.d == 0: flat tiles of C lines.
+    FastDiv vrow;
     int L;                 // FFT length of this pass
     int LP;                // padded LDS pitch of one line (complex elements)
     int C;                 // lines per tile (power of 2)

@@ -59,8 +59,10 @@ const DeviceInfo& device_info()
                                                       (int)info.lds_per_block);
         for (int prec : {4, 8})
             for (int L = 64; L <= 4096; L *= 2)
-                (void)hipFuncSetAttribute(kernel_symbol_fixed(prec, L), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                          (int)info.lds_per_block);
+                for (int t : {256, 512})
+                    if (const void* f = kernel_symbol_fixed(prec, L, t))
+                        (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                  (int)info.lds_per_block);
         for (int m : {kMfmaF32, kMfmaBf16x3, kMfmaF16x2})
             for (int L : {256, 4096})
                 for (int nb : {2, 4})
@@ -503,7 +505,9 @@ static bool build_pass(Plan* plan, TableBuilder& tb, const PassReq& q, std::stri
         for (int s = 0; plan_match && s < S; s++)
             plan_match = radix[s] == (s < S - 1 ? 16 : (1 << rem));
     }
-    const int tmax = plan_match ? 256 : 1024;
+    // fixed kernels come in 256- and 512-thread variants (512 for wide column
+    // tiles of L >= 256)
+    const int tmax = plan_match ? (q.L >= 256 ? 512 : 256) : 1024;
     // matrix-core pass: complex64, L = 16^2 or 16^3 with the plain 16-radix
     // plan, row mapping, no fused twiddle / real-packed step
     const int mm = plan->spec.mfma_mode;
@@ -551,10 +555,12 @@ static bool build_pass(Plan* plan, TableBuilder& tb, const PassReq& q, std::stri
     } else if (in_col || out_col) {
         const int cmax = plan->spec.tune.col_c ? plan->spec.tune.col_c : env_int("FFTZ_COL_C", prec == 8 ? 16 : 32);
         const size_t budget = std::min(lds_cap, (size_t)env_int("FFTZ_COL_LDS", 96 * 1024));
-        while (C * 2 <= cmax && (size_t)(C * 2) * line_bytes <= budget && (n0 % (C * 2)) == 0 &&
-               C * 2 * TL <= tmax)
+        // a row of n0 lines that C does not divide is tiled raggedly (the
+        // last tile of each row partly idle) rather than falling back to
+        // element-strided row tiles
+        while (C * 2 <= cmax && (size_t)(C * 2) * line_bytes <= budget && C * 2 <= n0 && C * 2 * TL <= tmax)
             C *= 2;
-        if (n0 % C != 0 || C < 2) { in_col = false; out_col = false; C = 1; }
+        if (C < 2) { in_col = false; out_col = false; C = 1; }
     }
     if (!(in_col || out_col) && !use_mfma) {
         const int tgt = plan->spec.tune.row_t ? plan->spec.tune.row_t : env_int("FFTZ_ROW_T", 256);
@@ -606,21 +612,35 @@ static bool build_pass(Plan* plan, TableBuilder& tb, const PassReq& q, std::stri
     ln.in_role = q.in_role;
     ln.out_role = q.out_role;
     ln.block = (uint32_t)(C * TL);
-    ln.maxt = ln.block <= 256 ? 256 : 1024;
-    ln.fixed = plan_match && ln.block <= 256 && !use_mfma;
+    ln.fixed = plan_match && ln.block <= 512 && !use_mfma;
+    ln.maxt = ln.block <= 256 ? 256 : (ln.fixed ? 512 : 1024);
     if (use_mfma) {
         ln.mfma = mm;
         ln.mfma_nb = env_int("FFTZ_MFMA_NB", 4) == 2 ? 2 : 4;
         ln.block = (uint32_t)(1024 / ln.mfma_nb);
         ln.frag_off = mfma_fragments(tb, mm);
     }
-    const int64_t tiles = (nl + C - 1) / C;
+    int64_t tiles = (nl + C - 1) / C;
+    if ((in_col || out_col) && n0 % C != 0) {
+        // ragged rows: ceil(n0 / C) tiles per row of n0 lines
+        const int64_t tpr = (n0 + C - 1) / C;
+        tiles = tpr * (nl / n0);
+        if (tiles * C >= ((int64_t)1 << 31)) {
+            err = "too many lines in one pass";
+            return false;
+        }
+        d.vrow = make_fastdiv((uint32_t)(tpr * C));
+    }
     ln.grid = (uint32_t)tiles;
     ln.lds_bytes = (uint32_t)((size_t)C * line_bytes);
     // XCD-contiguous tile order: measured to help passes that read rows and
     // store columns (the four-step's transposing last pass) and to cost ~5%
     // on passes that read columns
-    if (out_col && !in_col && tiles >= 64 && env_int("FFTZ_XCD_REMAP", 1)) d.flags |= kXcdRemap;
+    // Also for column tiles whose segments straddle cache lines (an element
+    // stride that is not a multiple of 128 bytes, e.g. the 257-wide
+    // half-complex rows of a 3-D R2C): neighbouring tiles share those lines.
+    const bool straddle = in_col && ((q.ies * esize) % 128) != 0;
+    if (((out_col && !in_col) || straddle) && tiles >= 64 && env_int("FFTZ_XCD_REMAP", 1)) d.flags |= kXcdRemap;
 
     // expand host-looped axes
     std::vector<int64_t> ioffs{q.in_base}, ooffs{q.out_base}, gis{0}, gos{0};
@@ -1849,6 +1869,7 @@ std::string plan_json(const Plan* p)
            << "," << d.ils[2] << "],\"ols\":[" << d.ols[0] << "," << d.ols[1] << "," << d.ols[2]
            << "],\"nlines\":" << d.nlines << ",\"ln0\":" << d.ln0.d << ",\"ln1\":" << d.ln1.d
            << ",\"flags\":" << d.flags << ",\"C\":" << d.C << ",\"TL\":" << d.TL << ",\"LP\":" << d.LP
+           << ",\"vrow\":" << d.vrow.d << ",\"maxt\":" << ln.maxt
            << ",\"E\":" << ln.E << ",\"fixed\":" << (ln.fixed ? 1 : 0) << ",\"mfma\":" << ln.mfma << ",\"grid\":" << ln.grid << ",\"block\":" << ln.block
            << ",\"lds\":" << ln.lds_bytes << ",\"in_role\":" << (int)ln.in_role
            << ",\"out_role\":" << (int)ln.out_role << ",\"in_off\":" << ln.in_off_bytes
@@ -1997,7 +2018,7 @@ static hipError_t launch_all(Plan* p, void* work, unsigned* counters, const void
         const float* frag = ln.frag;
         void* args[] = {&d, &frag};
         const void* fn = ln.mfma ? kernel_symbol_mfma(ln.mfma, ln.d.L, ln.mfma_nb)
-                         : ln.fixed ? kernel_symbol_fixed(p->spec.prec_bytes, ln.d.L)
+                         : ln.fixed ? kernel_symbol_fixed(p->spec.prec_bytes, ln.d.L, ln.maxt)
                                     : kernel_symbol(ln.kind, p->spec.prec_bytes, ln.all_radix, ln.E, ln.maxt);
         hipError_t e = hipLaunchKernel(fn, dim3(ln.grid),
                                        dim3(ln.block), args, ln.lds_bytes, stream);

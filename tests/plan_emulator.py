@@ -212,8 +212,21 @@ def check_launch_limits(plan, lds_cap=163840):
         need = L + (1 if ln["flags"] & K_C2R_PRE else 0)
         assert ln["LP"] >= (need - 1) + ((need - 1) >> 4) + 1
         assert ln["lds"] == C * ln["LP"] * (16 if plan["prec"] == 8 else 8)
-        ntiles = -(-ln["nlines"] // C)
         G = ln["grid"]
+        vrow = ln.get("vrow", 0)
+        if vrow:
+            # ragged column tiles: rows of ln0 lines, vrow / C tiles per row;
+            # every line is held by exactly one live slot
+            n0 = ln["ln0"]
+            assert vrow % C == 0 and vrow >= n0 > vrow - C and ln["nlines"] % n0 == 0
+            ntiles = (vrow // C) * (ln["nlines"] // n0)
+            v = np.arange(ntiles * C, dtype=np.int64)
+            row, cc = v // vrow, v % vrow
+            live = cc < n0
+            lines = (row * n0 + cc)[live]
+            assert np.array_equal(np.sort(lines), np.arange(ln["nlines"])), "ragged slots must cover each line once"
+        else:
+            ntiles = -(-ln["nlines"] // C)
         # one tile per workgroup; the XCD remap must be a permutation
         assert G == ntiles
         tiles = []

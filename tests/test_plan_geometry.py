@@ -262,3 +262,32 @@ def test_real_layouts_emulated(kind, n):
         back = np.zeros(nr)
         emu.run(plan_b, {0: out.copy(), 1: back, 2: work})
     assert O.rel_l2(back[RL.real_index(dims, vecs, n)] / n, x) < 1e-12
+
+
+@pytest.mark.parametrize("dims,vecs,prec", [
+    # column axis over rows of 257 lines (odd): ragged tiles of 16 / 8 lines
+    ([(257, 1, 1), (512, 257, 257)], [(2, 257 * 512, 257 * 512)], "f"),
+    ([(257, 1, 1), (512, 257, 257)], [(1, 257 * 512, 257 * 512)], "d"),
+    # 3-D with an odd innermost width and a mixed-radix column length
+    ([(33, 1, 1), (96, 33, 33), (40, 33 * 96, 33 * 96)], [(1, 33 * 96 * 40, 33 * 96 * 40)], "f"),
+])
+def test_ragged_column_tiles(dims, vecs, prec):
+    """Column passes over a row of lines that the tile width does not divide
+    use ragged tiles (each row's last tile partly idle) instead of
+    element-strided row tiles; the emulator checks every line is held by one
+    live slot and the result is the DFT."""
+    plan = c2c_case(dims, vecs, prec)
+    assert any(ln["vrow"] for ln in plan["launches"]), plan["launches"]
+
+
+def test_r2c_512_cubed_column_passes_are_ragged_and_wide():
+    """The 3-D R2C 512^3 plan: its two column passes run 16-line tiles (the
+    first over rows of 257 half-complex lines, raggedly) on 512-thread fixed
+    kernels."""
+    n, h = 512, 257
+    pj = fftz.plan_json([(n, 1, 1), (n, n, h), (n, n * n, n * h)], [(1, n ** 3, n * n * h)], "f", 0, fft_type=1)
+    emu.check_launch_limits(pj)
+    cols = pj["launches"][1:]
+    assert [ln["C"] for ln in cols] == [16, 16] and all(ln["flags"] & 24 == 24 for ln in cols)
+    assert cols[0]["vrow"] == 272 and cols[1]["vrow"] == 0
+    assert all(ln["fixed"] and ln["maxt"] == 512 for ln in cols)
