@@ -21,9 +21,11 @@
 //
 // Data path per tile (16 column blocks, 4 per wavefront):
 //   stage 0 gathers its operands straight from HBM;
-//   every stage's result is multiplied by the next stage's twiddle (FP32
-//   stage tables, w_{16 ls'}^(i' c')) and written to LDS;
-//   the last stage stores straight to HBM.
+//   later stages read theirs from LDS and multiply them by the stage twiddle
+//   (FP32 stage table, w_{16 ls}^(i c), loaded before the LDS reads);
+//   operands are split into hi / lo a pair at a time (packed conversions);
+//   every stage but the last writes its outputs to LDS in Stockham order,
+//   the last stores straight to HBM.
 // Hardware operand layout of the 16x16x32 forms: lane l holds A[l % 16][8 (l/16) + t]
 // and B[8 (l/16) + t][l % 16], t < 8; result D[4 (l/16) + r][l % 16], r < 4.
 // The 16x16x4 FP32 form: A[l % 16][l / 16], B[l / 16][l % 16], same D.  (Both
@@ -41,6 +43,41 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+
+// hi + lo split of 8 floats, a pair at a time (one packed conversion for
+// each half of each pair): hi = round(x), lo = round(x - hi).
+__device__ __forceinline__ void split_bf16(const float (&x)[8], bf16x8& hi, bf16x8& lo)
+{
+#pragma unroll
+    for (int t = 0; t < 8; t += 2) {
+        const f32x2 v = {x[t], x[t + 1]};
+        const bf16x2 h = __builtin_convertvector(v, bf16x2);
+        const f32x2 r = v - __builtin_convertvector(h, f32x2);
+        const bf16x2 l = __builtin_convertvector(r, bf16x2);
+        hi[t] = h[0];
+        hi[t + 1] = h[1];
+        lo[t] = l[0];
+        lo[t + 1] = l[1];
+    }
+}
+
+__device__ __forceinline__ void split_f16(const float (&x)[8], float sc, f16x8& hi, f16x8& lo)
+{
+#pragma unroll
+    for (int t = 0; t < 8; t += 2) {
+        const f32x2 v = {x[t] * sc, x[t + 1] * sc};
+        const f16x2 h = __builtin_convertvector(v, f16x2);
+        const f32x2 r = v - __builtin_convertvector(h, f32x2);
+        const f16x2 l = __builtin_convertvector(r, f16x2);
+        hi[t] = h[0];
+        hi[t + 1] = h[1];
+        lo[t] = l[0];
+        lo[t + 1] = l[1];
+    }
+}
 
 __device__ __forceinline__ void mfma_barrier()
 {
@@ -67,8 +104,6 @@ __device__ __forceinline__ LineOffM line_off_m(const PassDesc& p, uint32_t l)
     o.ooff = (int64_t)l0 * p.ols[0] + (int64_t)l1 * p.ols[1] + (int64_t)l2 * p.ols[2];
     return o;
 }
-
-__device__ __forceinline__ int padm(int e) { return e + (e >> 4); }
 
 // Operand layout (all modes).  The GEMM's K index is permuted so every lane
 // holds four whole complex inputs of its column: lane group g = lane / 16
@@ -144,13 +179,7 @@ __device__ __forceinline__ void dft16_blocks(const float (&x)[NB][8], const Frag
     } else if constexpr (MODE == kMfmaBf16x3) {
         bf16x8 xh[NB], xl[NB];
 #pragma unroll
-        for (int b = 0; b < NB; b++)
-#pragma unroll
-            for (int t = 0; t < 8; t++) {
-                const __bf16 h = (__bf16)x[b][t];
-                xh[b][t] = h;
-                xl[b][t] = (__bf16)(x[b][t] - (float)h);
-            }
+        for (int b = 0; b < NB; b++) split_bf16(x[b], xh[b], xl[b]);
 #pragma unroll
         for (int b = 0; b < NB; b++) {
             yr[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A.tl, xh[b], yr[b], 0, 0, 0);
@@ -183,13 +212,7 @@ __device__ __forceinline__ void dft16_blocks(const float (&x)[NB][8], const Frag
             e = e < -100 ? -100 : e;
             const float sc = ldexpf(1.0f, 14 - e);
             inv[b] = ldexpf(1.0f, e - 14);
-#pragma unroll
-            for (int t = 0; t < 8; t++) {
-                const float v = x[b][t] * sc;
-                const _Float16 h = (_Float16)v;
-                xh[b][t] = h;
-                xl[b][t] = (_Float16)(v - (float)h);
-            }
+            split_f16(x[b], sc, xh[b], xl[b]);
         }
 #pragma unroll
         for (int b = 0; b < NB; b++) {
@@ -219,7 +242,11 @@ __device__ __forceinline__ void dft16_blocks(const float (&x)[NB][8], const Frag
 }  // namespace
 
 // One workgroup per tile of C lines, C * L / 256 = 16 column blocks per stage,
-// NB per wavefront (16 / NB wavefronts per workgroup).
+// NB per wavefront (16 / NB wavefronts per workgroup).  Stage s (ls = 16^s)
+// reads butterfly j's inputs x[i * nbl + j], multiplies input i by the stage
+// twiddle w_{16 ls}^(i (j mod ls)) (table layout [j mod ls][i - 1], fetched
+// before the LDS reads), runs the DFT-16 GEMM and writes output k to
+// y[(j / ls) 16 ls + k ls + j mod ls] (Stockham order; HBM at the last stage).
 template <int MODE, int L, int NB>
 __global__ void __launch_bounds__(64 * 16 / NB) k_mfma_pass(const PassDesc p, const float* __restrict__ frag)
 {
@@ -227,6 +254,7 @@ __global__ void __launch_bounds__(64 * 16 / NB) k_mfma_pass(const PassDesc p, co
     constexpr int nbl = L / 16;           // butterflies per line per stage
     constexpr int CPL = L / 256;          // column blocks per line
     constexpr int LP = ((L - 1) + ((L - 1) >> 4) + 1) | 1;
+    constexpr int RSTRIDE = nbl + nbl / 16;  // LDS distance of inputs i and i + 1 (padded)
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     cplx<float>* lds = reinterpret_cast<cplx<float>*>(smem_raw);
     const int tid = threadIdx.x;
@@ -251,65 +279,82 @@ __global__ void __launch_bounds__(64 * 16 / NB) k_mfma_pass(const PassDesc p, co
     const float cj = (p.flags & kConjIn) ? -1.0f : 1.0f;
     const cplx<float>* twb = reinterpret_cast<const cplx<float>*>(p.stw);
 
+    // per-block constants: line c in the tile, butterfly j in the line
+    int bc[NB], bj[NB];
+    bool live[NB];
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+        const int blk = wv * NB + b;
+        bc[b] = blk / CPL;
+        bj[b] = (blk % CPL) * 16 + cl;
+        live[b] = (l_base + bc[b]) < p.nlines;
+    }
+
 #pragma unroll
     for (int s = 0; s < S; s++) {
         const int ls = s == 0 ? 1 : (s == 1 ? 16 : 256);
         float x[NB][8];
-        // gather: four complex inputs per lane per block
+        if (s == 0) {
 #pragma unroll
-        for (int b = 0; b < NB; b++) {
-            const int blk = wv * NB + b;
-            const int c = blk / CPL;
-            const int j = (blk % CPL) * 16 + cl;
-            cplx<float> v[4];
-            if (s == 0) {
-                if ((l_base + c) < p.nlines) {
-                    const int64_t ioff = line_off_m(p, (uint32_t)(l_base + c)).ioff;
-                    const cplx<float>* src = reinterpret_cast<const cplx<float>*>(p.in) + ioff + (int64_t)j * p.ies;
+            for (int b = 0; b < NB; b++) {
+                cplx<float> v[4] = {{0.f, 0.f}, {0.f, 0.f}, {0.f, 0.f}, {0.f, 0.f}};
+                if (live[b]) {
+                    const int64_t ioff = line_off_m(p, (uint32_t)(l_base + bc[b])).ioff;
+                    const cplx<float>* src = reinterpret_cast<const cplx<float>*>(p.in) + ioff + (int64_t)bj[b] * p.ies;
                     const int64_t rstep = (int64_t)nbl * p.ies;
 #pragma unroll
                     for (int q = 0; q < 4; q++) v[q] = src[rho(g, q) * rstep];
-                } else {
-#pragma unroll
-                    for (int q = 0; q < 4; q++) v[q] = {0.f, 0.f};
                 }
 #pragma unroll
-                for (int q = 0; q < 4; q++) v[q].y *= cj;
-            } else {
-                const cplx<float>* line = lds + c * LP;
+                for (int q = 0; q < 4; q++) {
+                    x[b][2 * q] = v[q].x;
+                    x[b][2 * q + 1] = v[q].y * cj;
+                }
+            }
+        } else {
+            // twiddles first: w_{16 ls}^(i cc) for the lane's inputs i = 4 q + g
+            cplx<float> w[NB][4];
 #pragma unroll
-                for (int q = 0; q < 4; q++) v[q] = line[padm(rho(g, q) * nbl + j)];
+            for (int b = 0; b < NB; b++) {
+                const int cc = bj[b] & (ls - 1);
+                const cplx<float>* tw = twb + p.twoff[s] + cc * 15 - 1;
+#pragma unroll
+                // (input 0 of lane group 0 reads the entry before the stage's
+                // table, still inside the arena since stage 0 precedes it,
+                // and uses 1 instead)
+                for (int q = 0; q < 4; q++) w[b][q] = tw[rho(g, q)];
+                if (g == 0) w[b][0] = {1.0f, 0.0f};
             }
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                x[b][2 * q] = v[q].x;
-                x[b][2 * q + 1] = v[q].y;
+            for (int b = 0; b < NB; b++) {
+                const cplx<float>* line = lds + bc[b] * LP + bj[b] + (bj[b] >> 4);
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const cplx<float> v = cmul(line[rho(g, q) * RSTRIDE], w[b][q]);
+                    x[b][2 * q] = v.x;
+                    x[b][2 * q + 1] = v.y;
+                }
             }
+            mfma_barrier();  // every read of this stage done before its writes
         }
-        if (s > 0) mfma_barrier();  // every read of this stage done before its writes
         f32x4 yr[NB], yi[NB];
         dft16_blocks<MODE, NB>(x, A, yr, yi);
 #pragma unroll
         for (int b = 0; b < NB; b++) {
-            const int blk = wv * NB + b;
-            const int c = blk / CPL;
-            const int j = (blk % CPL) * 16 + cl;
-            const int gg = j / ls, cc = j % ls;
+            const int j = bj[b];
             if (s < S - 1) {
-                cplx<float>* line = lds + c * LP;
-                const int ls2 = ls * 16;
+                // output k = 4 g + r to y[gg 16 ls + k ls + cc]; padded index
+                // pos + pos / 16 is linear in r
+                const int gg = j / ls, cc = j & (ls - 1);
+                const int pos0 = gg * 16 * ls + 4 * g * ls + cc;
+                cplx<float>* line = lds + bc[b] * LP;
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
-                    const int k = 4 * g + r;
-                    const int pos = gg * 16 * ls + k * ls + cc;
-                    // next stage: pos = i2 * nbl + j2, twiddle w_{16 ls2}^(i2 * (j2 % ls2))
-                    const int i2 = pos / nbl, j2 = pos % nbl, c2 = j2 % ls2;
-                    cplx<float> y = {yr[b][r], yi[b][r]};
-                    if (i2 > 0 && c2 > 0) y = cmul(y, twb[p.twoff[s + 1] + c2 * 15 + (i2 - 1)]);
-                    line[padm(pos)] = y;
+                    const int pos = pos0 + r * ls;
+                    line[pos + (pos >> 4)] = cplx<float>{yr[b][r], yi[b][r]};
                 }
-            } else if ((l_base + c) < p.nlines) {
-                const int64_t ooff = line_off_m(p, (uint32_t)(l_base + c)).ooff;
+            } else if (live[b]) {
+                const int64_t ooff = line_off_m(p, (uint32_t)(l_base + bc[b])).ooff;
                 cplx<float>* dst = reinterpret_cast<cplx<float>*>(p.out) + ooff + (int64_t)j * p.oes;
                 const int64_t rstep = (int64_t)nbl * p.oes;
 #pragma unroll
