@@ -155,9 +155,12 @@ void* plan_guru(int rank, const IOD* dims, int hrank, const IOD* hdims, void* in
     return make_plan<R>(dv, in, out, sign, kind);
 }
 
-void* unsupported_r2r()
+// Real-to-real planners and execute: unsupported, as in the reference
+// (wrappers/fftw/src/c/fftw_plan_wrapper.c:457-560, fftw_execute_wrapper.c:59-73,
+// with the warning of translator/fftz_translator.h:54-57); planners return NULL.
+void* unsupported_r2r(const char* func)
 {
-    fprintf(stderr, "[ERROR] : aoclfftz : fftw r2r : real-to-real transforms are not provided\n");
+    fprintf(stderr, "[AOCL-FFTZ] WARNING: %s: r2r transforms are not supported\n", func);
     return nullptr;
 }
 
@@ -340,17 +343,39 @@ char* sprint(void* p)
     {                                                                                                      \
         return plan_guru<R>(rank, dims, hr, hd, in, out, FFTW_BACKWARD, C2R);                              \
     }                                                                                                      \
-    X##plan X##plan_r2r_1d(int, R*, R*, X##r2r_kind, unsigned) { return unsupported_r2r(); }               \
+    X##plan X##plan_r2r(int, const int*, R*, R*, const X##r2r_kind*, unsigned)                               \
+    {                                                                                                      \
+        return unsupported_r2r(__func__);                                                                  \
+    }                                                                                                      \
+    X##plan X##plan_r2r_1d(int, R*, R*, X##r2r_kind, unsigned) { return unsupported_r2r(__func__); }       \
+    X##plan X##plan_r2r_2d(int, int, R*, R*, X##r2r_kind, X##r2r_kind, unsigned)                           \
+    {                                                                                                      \
+        return unsupported_r2r(__func__);                                                                  \
+    }                                                                                                      \
+    X##plan X##plan_r2r_3d(int, int, int, R*, R*, X##r2r_kind, X##r2r_kind, X##r2r_kind, unsigned)         \
+    {                                                                                                      \
+        return unsupported_r2r(__func__);                                                                  \
+    }                                                                                                      \
     X##plan X##plan_many_r2r(int, const int*, int, R*, const int*, int, int, R*, const int*, int, int,      \
                              const X##r2r_kind*, unsigned)                                                 \
     {                                                                                                      \
-        return unsupported_r2r();                                                                          \
+        return unsupported_r2r(__func__);                                                                  \
+    }                                                                                                      \
+    X##plan X##plan_guru_r2r(int, const X##iodim*, int, const X##iodim*, R*, R*, const X##r2r_kind*,       \
+                             unsigned)                                                                     \
+    {                                                                                                      \
+        return unsupported_r2r(__func__);                                                                  \
+    }                                                                                                      \
+    X##plan X##plan_guru64_r2r(int, const X##iodim64*, int, const X##iodim64*, R*, R*,                     \
+                               const X##r2r_kind*, unsigned)                                               \
+    {                                                                                                      \
+        return unsupported_r2r(__func__);                                                                  \
     }                                                                                                      \
     void X##execute(const X##plan p) { (void)aoclfftz_execute(p); }                                        \
     void X##execute_dft(const X##plan p, C* in, C* out) { (void)aoclfftz_execute_io(p, in, out); }         \
     void X##execute_dft_r2c(const X##plan p, R* in, C* out) { (void)aoclfftz_execute_io(p, in, out); }     \
     void X##execute_dft_c2r(const X##plan p, C* in, R* out) { (void)aoclfftz_execute_io(p, in, out); }     \
-    void X##execute_r2r(const X##plan, R*, R*) { (void)unsupported_r2r(); }                                \
+    void X##execute_r2r(const X##plan, R*, R*) { (void)unsupported_r2r(__func__); }                        \
     void X##destroy_plan(X##plan p) { aoclfftz_destroy(p); }                                               \
     void X##cleanup(void) {}                                                                               \
     void X##set_timelimit(double) {}                                                                       \

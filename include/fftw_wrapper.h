@@ -10,7 +10,9 @@
  * Buffers may be host memory (fftw_malloc) or device memory; plans created
  * here execute synchronously like FFTW.  Planner flags (FFTW_MEASURE, ...)
  * are accepted and ignored, as in the reference.  Real-to-real plans
- * (r2r) are not provided: their planners return NULL.
+ * (r2r) are unsupported, as in the reference (wrappers/fftw/src/c/
+ * fftw_plan_wrapper.c:457-560): their planners print a warning and return
+ * NULL, and execute_r2r does nothing.
  */
 #ifndef FFTW_WRAPPER_H
 #define FFTW_WRAPPER_H
@@ -35,6 +37,19 @@ extern "C" {
 #define FFTW_PATIENT (1U << 5)
 #define FFTW_ESTIMATE (1U << 6)
 #define FFTW_WISDOM_ONLY (1U << 21)
+
+/* r2r kinds (planners unsupported, as in the reference) */
+#define FFTW_R2HC 0
+#define FFTW_HC2R 1
+#define FFTW_DHT 2
+#define FFTW_REDFT00 3
+#define FFTW_REDFT01 4
+#define FFTW_REDFT10 5
+#define FFTW_REDFT11 6
+#define FFTW_RODFT00 7
+#define FFTW_RODFT01 8
+#define FFTW_RODFT10 9
+#define FFTW_RODFT11 10
 
 #if defined(_Complex_I) && defined(complex) && defined(I) && !defined(FFTW_NO_Complex)
 typedef double _Complex fftw_complex;
@@ -93,7 +108,18 @@ typedef fftw_read_char_func fftwf_read_char_func;
                                  const X##iodim *howmany_dims, C *in, R *out, unsigned flags);          \
     X##plan X##plan_guru64_dft_c2r(int rank, const X##iodim64 *dims, int howmany_rank,                  \
                                    const X##iodim64 *howmany_dims, C *in, R *out, unsigned flags);      \
+    X##plan X##plan_r2r(int rank, const int *n, R *in, R *out, const X##r2r_kind *kind, unsigned flags); \
     X##plan X##plan_r2r_1d(int n, R *in, R *out, X##r2r_kind kind, unsigned flags);                    \
+    X##plan X##plan_r2r_2d(int n0, int n1, R *in, R *out, X##r2r_kind kind0, X##r2r_kind kind1,        \
+                           unsigned flags);                                                             \
+    X##plan X##plan_r2r_3d(int n0, int n1, int n2, R *in, R *out, X##r2r_kind kind0,                   \
+                           X##r2r_kind kind1, X##r2r_kind kind2, unsigned flags);                       \
+    X##plan X##plan_guru_r2r(int rank, const X##iodim *dims, int howmany_rank,                          \
+                             const X##iodim *howmany_dims, R *in, R *out, const X##r2r_kind *kind,      \
+                             unsigned flags);                                                           \
+    X##plan X##plan_guru64_r2r(int rank, const X##iodim64 *dims, int howmany_rank,                      \
+                               const X##iodim64 *howmany_dims, R *in, R *out,                           \
+                               const X##r2r_kind *kind, unsigned flags);                                \
     X##plan X##plan_many_r2r(int rank, const int *n, int howmany, R *in, const int *inembed,           \
                              int istride, int idist, R *out, const int *onembed, int ostride,           \
                              int odist, const X##r2r_kind *kind, unsigned flags);                       \

@@ -34,6 +34,13 @@ def declared_functions():
                 if m.group(1) not in ("plan", "iodim", "iodim64", "r2r_kind", "write_char_func", "read_char_func"):
                     names.add("fftw_" + m.group(1))
                     names.add("fftwf_" + m.group(1))
+        # legacy Fortran subroutines, generated per name prefix: F##name(...)
+        if "FFTZ_FORTRAN_DECLARE" in src:
+            body = src[src.index("#define FFTZ_FORTRAN_DECLARE"):src.index("FFTZ_FORTRAN_DECLARE(dfftw_")]
+            prefixes = re.findall(r"^FFTZ_FORTRAN_DECLARE\((\w+),", src, flags=re.M)
+            for m in re.finditer(r"F##(\w+)\s*\(", body):
+                for pre in prefixes:
+                    names.add(pre + m.group(1))
     return sorted(names)
 
 
@@ -44,6 +51,8 @@ def test_library_exports_every_declared_symbol():
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
     assert "fftw_plan_many_dft" in names and "fftwf_execute_dft" in names and "fftw_version" in names
+    assert "dfftw_plan_many_dft_" in names and "dfftwf_execute_dft_" in names and "sfftw_destroy_plan_" in names
+    assert "fftw_plan_r2r_2d" in names and "fftwf_plan_guru64_r2r" in names
     assert set(fftz.EXPORTS) <= set(names)
 
 
@@ -147,3 +156,15 @@ def test_control_fields_normalised_not_rejected():
     assert d.cntrl_params.logger_mode == 0
     assert d.cntrl_params.measure_stats == 0
     assert d.cntrl_params.opt_level == 3
+
+
+@pytest.mark.parametrize("prog", ["legacy.f", "modern.f90"])
+def test_fortran_callers_compile_and_link(prog, tmp_path):
+    """The Fortran interface files (include/fftw_wrapper.f, fftw_wrapper.f03)
+    compile with the image's flang and every subroutine / function the two
+    caller programs use resolves in the library (running them needs a GPU:
+    tests/test_gpu_fortran.py)."""
+    from fortran_util import build_fortran, flang
+    if flang() is None:
+        pytest.skip("no Fortran compiler in this image")
+    assert os.path.exists(build_fortran(prog, tmp_path))
