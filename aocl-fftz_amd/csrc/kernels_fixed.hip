@@ -80,6 +80,27 @@ __device__ __forceinline__ void lds_barrier()
     asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
+// Stage exchange of a pass whose lines each live in one wavefront (row
+// mapping on both sides, TL <= 64 threads per line): the wave's own LDS
+// operations complete in order once counted down, so waiting for them is
+// enough and the wavefronts of a workgroup never wait for each other.
+__device__ __forceinline__ void wave_lds_sync()
+{
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+}
+
+template <int TL>
+__device__ __forceinline__ void stage_sync(bool wave_local)
+{
+    if constexpr (TL <= 64) {
+        if (wave_local) {
+            wave_lds_sync();
+            return;
+        }
+    }
+    lds_barrier();
+}
+
 template <typename R, int RAD>
 __device__ __forceinline__ void twiddle_legs(cplx<R>* v, const cplx<R>* w)
 {
@@ -92,7 +113,7 @@ __device__ __forceinline__ void twiddle_legs(cplx<R>* v, const cplx<R>* w)
 // One LDS stage s (not the first, not the last): read K butterflies, barrier,
 // twiddle + DFT, write, barrier.  (c, t) is this stage's thread mapping.
 template <typename R, int L, int s>
-__device__ __forceinline__ void fixed_mid_stage(const PassDesc& p, cplx<R>* lds, int c, int t)
+__device__ __forceinline__ void fixed_mid_stage(const PassDesc& p, cplx<R>* lds, int c, int t, bool wave_local)
 {
     using P = Plan2<L>;
     constexpr int RAD = P::radix(s);
@@ -109,7 +130,7 @@ __device__ __forceinline__ void fixed_mid_stage(const PassDesc& p, cplx<R>* lds,
 #pragma unroll
         for (int i = 0; i < RAD; i++) v[k][i] = line[pad_c<L>(i * nbl + j)];
     }
-    lds_barrier();
+    stage_sync<TL>(wave_local);
     const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(p.stw) + p.twoff[s];
 #pragma unroll
     for (int k = 0; k < K; k++) {
@@ -122,16 +143,16 @@ __device__ __forceinline__ void fixed_mid_stage(const PassDesc& p, cplx<R>* lds,
 #pragma unroll
         for (int i = 0; i < RAD; i++) line[pad_c<L>(base + i * ls)] = v[k][i];
     }
-    lds_barrier();
+    stage_sync<TL>(wave_local);
 }
 
 template <typename R, int L, int s, int S>
 struct MidStages {
-    static __device__ __forceinline__ void run(const PassDesc& p, cplx<R>* lds, int c, int t)
+    static __device__ __forceinline__ void run(const PassDesc& p, cplx<R>* lds, int c, int t, bool wave_local)
     {
         if constexpr (s < S - 1) {
-            fixed_mid_stage<R, L, s>(p, lds, c, t);
-            MidStages<R, L, s + 1, S>::run(p, lds, c, t);
+            fixed_mid_stage<R, L, s>(p, lds, c, t, wave_local);
+            MidStages<R, L, s + 1, S>::run(p, lds, c, t, wave_local);
         }
     }
 };
@@ -194,10 +215,13 @@ __global__ void __launch_bounds__(MAXT) k_fixed_pass(const PassDesc p)
 #pragma unroll
         for (int i = 0; i < 16; i++) line[pad_c<L>(ti * 16 + i)] = v[i];
     }
-    lds_barrier();
+    // lines held by one wavefront each (row mapping throughout, TL <= 64)
+    // exchange through LDS without workgroup barriers
+    const bool wave_local = !(p.flags & (kInCol | kOutCol));
+    stage_sync<TL>(wave_local);
 
     // middle stages (row mapping)
-    MidStages<R, L, 1, S>::run(p, lds, tid / TL, tid % TL);
+    MidStages<R, L, 1, S>::run(p, lds, tid / TL, tid % TL, wave_local);
 
     // last stage (output-side mapping) + store
     const bool out_col = (p.flags & kOutCol) != 0;

@@ -64,11 +64,11 @@ __device__ __forceinline__ void split_bf16(const float (&x)[8], bf16x8& hi, bf16
     }
 }
 
-__device__ __forceinline__ void split_f16(const float (&x)[8], float sc, f16x8& hi, f16x8& lo)
+__device__ __forceinline__ void split_f16(const float (&x)[8], f16x8& hi, f16x8& lo)
 {
 #pragma unroll
     for (int t = 0; t < 8; t += 2) {
-        const f32x2 v = {x[t] * sc, x[t + 1] * sc};
+        const f32x2 v = {x[t], x[t + 1]};
         const f16x2 h = __builtin_convertvector(v, f16x2);
         const f32x2 r = v - __builtin_convertvector(h, f32x2);
         const f16x2 l = __builtin_convertvector(r, f16x2);
@@ -196,24 +196,11 @@ __device__ __forceinline__ void dft16_blocks(const float (&x)[NB][8], const Frag
             yi[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A.bh, xh[b], yi[b], 0, 0, 0);
         }
     } else {
-        // per-column power-of-two scale: a column's 32 values sit in lanes
-        // l, l^16, l^32, l^48
+        // operands already scaled into fp16 range by the caller (see the
+        // kernel: per-line power of two, stage twiddles carrying 1/16)
         f16x8 xh[NB], xl[NB];
-        float inv[NB];
 #pragma unroll
-        for (int b = 0; b < NB; b++) {
-            float m = 0.f;
-#pragma unroll
-            for (int t = 0; t < 8; t++) m = fmaxf(m, fabsf(x[b][t]));
-            m = fmaxf(m, __shfl_xor(m, 16));
-            m = fmaxf(m, __shfl_xor(m, 32));
-            int e = 0;
-            (void)frexpf(m, &e);  // m < 2^e
-            e = e < -100 ? -100 : e;
-            const float sc = ldexpf(1.0f, 14 - e);
-            inv[b] = ldexpf(1.0f, e - 14);
-            split_f16(x[b], sc, xh[b], xl[b]);
-        }
+        for (int b = 0; b < NB; b++) split_f16(x[b], xh[b], xl[b]);
 #pragma unroll
         for (int b = 0; b < NB; b++) {
             yr[b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A.tl, xh[b], yr[b], 0, 0, 0);
@@ -229,14 +216,24 @@ __device__ __forceinline__ void dft16_blocks(const float (&x)[NB][8], const Frag
             yr[b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A.th, xh[b], yr[b], 0, 0, 0);
             yi[b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A.bh, xh[b], yi[b], 0, 0, 0);
         }
-#pragma unroll
-        for (int b = 0; b < NB; b++)
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                yr[b][r] *= inv[b];
-                yi[b][r] *= inv[b];
-            }
     }
+}
+
+// Largest |value| over the wavefront's 64 lanes.
+__device__ __forceinline__ float wave_max(float m)
+{
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) m = fmaxf(m, __shfl_xor(m, o));
+    return m;
+}
+
+// 2^(14 - e) for m < 2^e: maps a line's largest component to below 2^14.
+__device__ __forceinline__ float fp16_scale(float m)
+{
+    int e = 0;
+    (void)frexpf(m, &e);
+    e = e < -100 ? -100 : (e > 100 ? 100 : e);
+    return ldexpf(1.0f, 14 - e);
 }
 
 }  // namespace
@@ -277,11 +274,16 @@ __global__ void __launch_bounds__(64 * 16 / NB) k_mfma_pass(const PassDesc p, co
         A.init(af);
     }
     const float cj = (p.flags & kConjIn) ? -1.0f : 1.0f;
-    const cplx<float>* twb = reinterpret_cast<const cplx<float>*>(p.stw);
+    // lane-ordered stage twiddles (p.rtw): stage s >= 1 at offset
+    // 16 (16 + ... + 16^(s-1)), entry [c][g][q] = w_{16 ls}^((4 q + g) c)
+    const float4* twq = reinterpret_cast<const float4*>(p.rtw);
 
     // per-block constants: line c in the tile, butterfly j in the line
     int bc[NB], bj[NB];
     bool live[NB];
+    float lsc[NB];  // fp16 operands: the block's line scale
+#pragma unroll
+    for (int b = 0; b < NB; b++) lsc[b] = 1.0f;
 #pragma unroll
     for (int b = 0; b < NB; b++) {
         const int blk = wv * NB + b;
@@ -300,10 +302,13 @@ __global__ void __launch_bounds__(64 * 16 / NB) k_mfma_pass(const PassDesc p, co
                 cplx<float> v[4] = {{0.f, 0.f}, {0.f, 0.f}, {0.f, 0.f}, {0.f, 0.f}};
                 if (live[b]) {
                     const int64_t ioff = line_off_m(p, (uint32_t)(l_base + bc[b])).ioff;
-                    const cplx<float>* src = reinterpret_cast<const cplx<float>*>(p.in) + ioff + (int64_t)bj[b] * p.ies;
                     const int64_t rstep = (int64_t)nbl * p.ies;
+                    // rows 4 q + g: one base for the lane group, then 4-row steps
+                    const cplx<float>* src =
+                        reinterpret_cast<const cplx<float>*>(p.in) + ioff + (int64_t)bj[b] * p.ies + g * rstep;
+                    const int64_t step4 = 4 * rstep;
 #pragma unroll
-                    for (int q = 0; q < 4; q++) v[q] = src[rho(g, q) * rstep];
+                    for (int q = 0; q < 4; q++) v[q] = src[q * step4];
                 }
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
@@ -311,19 +316,62 @@ __global__ void __launch_bounds__(64 * 16 / NB) k_mfma_pass(const PassDesc p, co
                     x[b][2 * q + 1] = v[q].y * cj;
                 }
             }
+            if constexpr (MODE == kMfmaF16x2) {
+                // Scaled fp16 operands: every line is scaled by a power of two
+                // that maps its largest component below 2^14 (exact, undone at
+                // the last store); later stages' twiddles carry 1/16, so the
+                // inputs of a stage stay below 2^14 * sqrt(2)^s, inside fp16.
+                // A line is one block (L = 256) or the whole tile (L = 4096).
+                if constexpr (CPL == 1) {
+#pragma unroll
+                    for (int b = 0; b < NB; b++) {
+                        float m = 0.f;
+#pragma unroll
+                        for (int t = 0; t < 8; t++) m = fmaxf(m, fabsf(x[b][t]));
+                        lsc[b] = fp16_scale(wave_max(m));
+                    }
+                } else {
+                    float m = 0.f;
+#pragma unroll
+                    for (int b = 0; b < NB; b++)
+#pragma unroll
+                        for (int t = 0; t < 8; t++) m = fmaxf(m, fabsf(x[b][t]));
+                    m = wave_max(m);
+                    float* slot = reinterpret_cast<float*>(lds + p.C * LP);  // past the tile
+                    if (lane == 0) slot[wv] = m;
+                    mfma_barrier();
+                    for (int w2 = 0; w2 < 16 / NB; w2++) m = fmaxf(m, slot[w2]);
+                    const float sc = fp16_scale(m);
+#pragma unroll
+                    for (int b = 0; b < NB; b++) lsc[b] = sc;
+                }
+#pragma unroll
+                for (int b = 0; b < NB; b++)
+#pragma unroll
+                    for (int t = 0; t < 8; t++) x[b][t] *= lsc[b];
+            }
         } else {
-            // twiddles first: w_{16 ls}^(i cc) for the lane's inputs i = 4 q + g
+            // twiddles first: w_{16 ls}^(i cc) for the lane's inputs i = 4 q + g,
+            // 32 contiguous bytes per block (one set for all blocks at stage
+            // 1, where cc = j mod 16 is the lane's column)
             cplx<float> w[NB][4];
+            // stage offset in rows of 16 entries: stage 1 has 16 rows (c < 16),
+            // stage 2 follows it
+            const int soff = s == 1 ? 0 : 16;
 #pragma unroll
             for (int b = 0; b < NB; b++) {
-                const int cc = bj[b] & (ls - 1);
-                const cplx<float>* tw = twb + p.twoff[s] + cc * 15 - 1;
+                if (s == 1 && b > 0) {
 #pragma unroll
-                // (input 0 of lane group 0 reads the entry before the stage's
-                // table, still inside the arena since stage 0 precedes it,
-                // and uses 1 instead)
-                for (int q = 0; q < 4; q++) w[b][q] = tw[rho(g, q)];
-                if (g == 0) w[b][0] = {1.0f, 0.0f};
+                    for (int q = 0; q < 4; q++) w[b][q] = w[0][q];
+                    continue;
+                }
+                const int cc = bj[b] & (ls - 1);
+                const float4* tq = twq + ((soff + cc) * 16 + 4 * g) / 2;
+                const float4 a0 = tq[0], a1 = tq[1];
+                w[b][0] = {a0.x, a0.y};
+                w[b][1] = {a0.z, a0.w};
+                w[b][2] = {a1.x, a1.y};
+                w[b][3] = {a1.z, a1.w};
             }
 #pragma unroll
             for (int b = 0; b < NB; b++) {
@@ -355,14 +403,19 @@ __global__ void __launch_bounds__(64 * 16 / NB) k_mfma_pass(const PassDesc p, co
                 }
             } else if (live[b]) {
                 const int64_t ooff = line_off_m(p, (uint32_t)(l_base + bc[b])).ooff;
-                cplx<float>* dst = reinterpret_cast<cplx<float>*>(p.out) + ooff + (int64_t)j * p.oes;
                 const int64_t rstep = (int64_t)nbl * p.oes;
+                // outputs 4 g + r (last stage: gg = 0, cc = j)
+                cplx<float>* dst =
+                    reinterpret_cast<cplx<float>*>(p.out) + ooff + (int64_t)j * p.oes + (int64_t)(4 * g) * rstep;
+                // undo the fp16 line scale and the 1/16 of each later stage
+                const float unscale = MODE == kMfmaF16x2 ? (S == 3 ? 256.0f : 16.0f) / lsc[b] : 1.0f;
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
                     cplx<float> y = {yr[b][r], yi[b][r]};
+                    if constexpr (MODE == kMfmaF16x2) y = cscale(y, unscale);
                     if (p.flags & kScale) y = cscale(y, p.scale_f);
                     if (p.flags & kConjOut) y.y = -y.y;
-                    dst[(4 * g + r) * rstep] = y;  // last stage: gg = 0, cc = j
+                    dst[r * rstep] = y;
                 }
             }
         }

@@ -86,7 +86,8 @@ struct PassDesc {
     const void* tw4;       // four-step twiddles: double2 lo[1<<tw4_lobits] then hi[]
     const void* chirp;     // Bluestein: conj(B_k), k < nvalid (plan precision)
     const void* spec;      // Bluestein: conj(FFT_M(B)) / M (plan precision)
-    const void* rtw;       // real-packed twiddles exp(-2 pi i k / 2L), k <= L/2
+    const void* rtw;       // real-packed twiddles exp(-2 pi i k / 2L), k <= L/2;
+                           // matrix-core pass: its lane-ordered stage twiddles
     int64_t ies, oes;      // element strides along the transform axis
     int64_t ils[3], ols[3];// line strides for line levels l0, l1, l2
     int64_t nlines;        // total lines

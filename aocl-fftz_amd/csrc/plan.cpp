@@ -430,6 +430,29 @@ static int64_t mfma_fragments(TableBuilder& tb, int mode)
     return tb.add(f.data(), f.size() * sizeof(float));
 }
 
+// Stage twiddles of the matrix-core pass in the order its lanes read them:
+// for stage s >= 1 (ls = 16^s) and c = j mod ls, 16 entries [g][q] =
+// w_{16 ls}^((4 q + g) c) -- lane group g's four inputs i = 4 q + g of
+// butterfly j, entry i = 0 being 1 -- stage after stage (FP64 roots rounded
+// to FP32).  A lane reads its four twiddles of a block as 32 contiguous bytes.
+// With scaled fp16 operands the entries carry an exact factor 1/16, which
+// keeps each stage's inputs inside the fp16 range (the kernel undoes it at
+// the last store).
+static int64_t mfma_stage_twiddles(TableBuilder& tb, int L, int mode)
+{
+    std::vector<d2> t;
+    const int S = L == 256 ? 2 : 3;
+    const double f = mode == kMfmaF16x2 ? 1.0 / 16.0 : 1.0;
+    for (int s = 1, ls = 16; s < S; s++, ls *= 16)
+        for (int c = 0; c < ls; c++)
+            for (int g = 0; g < 4; g++)
+                for (int q = 0; q < 4; q++) {
+                    const d2 w = root_fwd((int64_t)(4 * q + g) * c, 16 * ls);
+                    t.push_back({w.x * f, w.y * f});
+                }
+    return tb.add_complex(t, 4);
+}
+
 static bool build_pass(Plan* plan, TableBuilder& tb, const PassReq& q, std::string& err)
 {
     const int prec = plan->spec.prec_bytes;
@@ -619,6 +642,7 @@ static bool build_pass(Plan* plan, TableBuilder& tb, const PassReq& q, std::stri
         ln.mfma_nb = env_int("FFTZ_MFMA_NB", 4) == 2 ? 2 : 4;
         ln.block = (uint32_t)(1024 / ln.mfma_nb);
         ln.frag_off = mfma_fragments(tb, mm);
+        ln.rtw_off = mfma_stage_twiddles(tb, q.L, mm);  // the pass's d.rtw: see kernels_mfma.hip
     }
     int64_t tiles = (nl + C - 1) / C;
     if ((in_col || out_col) && n0 % C != 0) {
@@ -633,6 +657,7 @@ static bool build_pass(Plan* plan, TableBuilder& tb, const PassReq& q, std::stri
     }
     ln.grid = (uint32_t)tiles;
     ln.lds_bytes = (uint32_t)((size_t)C * line_bytes);
+    if (use_mfma) ln.lds_bytes += 64;  // per-wavefront maxima of the fp16 line scale
     // XCD-contiguous tile order: measured to help passes that read rows and
     // store columns (the four-step's transposing last pass) and to cost ~5%
     // on passes that read columns

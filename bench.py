@@ -332,6 +332,17 @@ def bench_batched_n1024(torch, fftz, steps, warmup, n=1024, b=4096):
     t1.record(stream)
     torch.cuda.synchronize()
     ms_warm = t0.elapsed_time(t1) / reps
+    # the same bytes moved by a plain device copy, same buffer rotation and
+    # back-to-back timing: the practical ceiling for a 32 MiB -> 32 MiB launch
+    for i in range(nbuf):
+        ys[i].copy_(xs[i])
+    torch.cuda.synchronize()
+    t0.record(stream)
+    for i in range(reps):
+        ys[i % nbuf].copy_(xs[i % nbuf])
+    t1.record(stream)
+    torch.cuda.synchronize()
+    ms_copy = t0.elapsed_time(t1) / reps
     _, avg, achieved = roofline_from_profile(prof, bpl)
     flops = 5.0 * n * math.log2(n) * b
     byts = 2 * 8.0 * n * b
@@ -344,6 +355,9 @@ def bench_batched_n1024(torch, fftz, steps, warmup, n=1024, b=4096):
         "kernel_achieved_Bps": achieved,
         "ms_per_execute_cache_warm": ms_warm,
         "gflops_cache_warm": flops / (ms_warm * 1e-3) / 1e9,
+        "ms_copy_same_bytes": ms_copy,
+        "copy_same_bytes_frac_of_hbm": byts / (ms_copy * 1e-3) / HBM_PEAK,
+        "frac_of_copy_same_bytes": ms_copy / ms_cold,
         "plan": p.info,
     }
     p.execute_io(xs[0], ys[0])
@@ -601,7 +615,7 @@ def bench_split(torch, fftz, steps, warmup, n=4096, b=16384):
     return out
 
 
-def bench_serving(torch, fftz, rates=(2000, 10000, 50000), seconds=0.4, max_count=4000, inflight=16):
+def bench_serving(torch, fftz, rates=(2000, 10000, 25000, 50000), seconds=0.4, max_count=4000, inflight=16):
     """Serving: open-loop Poisson arrivals of the serving request mix (1-D C2C,
     2-D R2C, 3-D C2C of seeded non-power-of-two sizes, 10% complex128) through
     one server (16 slots, 16 GiB budget); request buffers are carved from a

@@ -211,7 +211,8 @@ def check_launch_limits(plan, lds_cap=163840):
         assert ln["lds"] <= lds_cap
         need = L + (1 if ln["flags"] & K_C2R_PRE else 0)
         assert ln["LP"] >= (need - 1) + ((need - 1) >> 4) + 1
-        assert ln["lds"] == C * ln["LP"] * (16 if plan["prec"] == 8 else 8)
+        # (matrix-core passes keep 64 bytes past the tile for their fp16 line scale)
+        assert ln["lds"] == C * ln["LP"] * (16 if plan["prec"] == 8 else 8) + (64 if ln.get("mfma") else 0)
         G = ln["grid"]
         vrow = ln.get("vrow", 0)
         if vrow:

@@ -3,7 +3,8 @@ on complex64 lengths 256 and 4096, against the FP64 oracle.
 
 Relative-L2 bounds, stated per operand precision:
   FP32 operands (mode 1) and scaled fp16 hi+lo (mode 3): 2e-7 * log2 N, the
-  plain FP32 bound (fp16 pairs carry 22 significant bits, scaled per column);
+  plain FP32 bound (fp16 pairs carry 22 significant bits; each line is scaled
+  by a power of two into the fp16 range);
   bf16 hi+lo (mode 2): 1e-5 * log2 N (16 significant bits per operand, the
   lo*lo product dropped).
 """
@@ -124,3 +125,19 @@ def test_mfma_full_size_4096x16384(torch_gpu):
         assert rt <= 2 * mode_tol(n, mode)
         f.destroy()
         r.destroy()
+
+
+@pytest.mark.parametrize("n,b", [(256, 37), (4096, 5)])
+def test_mfma_fp16_lines_of_very_different_magnitude(torch_gpu, n, b):
+    """Scaled fp16 operands are scaled per line: lines whose magnitudes span
+    1e-12 .. 1e+12 each meet the FP32 bound on their own (a per-tile or global
+    scale would leave the small lines to fp16 subnormals)."""
+    x = rand_c((b, n), 77, "f")
+    mags = np.logspace(-12, 12, b).astype(np.float32)
+    x = (x * mags[:, None]).astype(np.complex64)
+    y, info = run(torch_gpu, x, 3)
+    assert MODES[3] in info, info
+    ref = O.fft(x.astype(np.complex128))
+    worst = max(O.rel_l2(y[i], ref[i]) for i in range(b))
+    print(f"n={n}: worst per-line rel-L2 {worst:.3e}")
+    assert worst <= tol(n, "f")
