@@ -59,6 +59,8 @@ def build(jobs: int = 8, incremental: bool = False) -> str:
     os.makedirs(OBJ, exist_ok=True)
     os.makedirs(LIBDIR, exist_ok=True)
     srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")) + glob.glob(os.path.join(CSRC, "*.cpp")))
+    # the generic-kernel units take minutes each: start them first
+    srcs.sort(key=lambda p: 0 if os.path.basename(p).startswith(("kernels_batch_", "kernels_pass_")) else 1)
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(lambda src: _compile(src, incremental), srcs))
     tmp = LIB + ".tmp"

@@ -181,6 +181,7 @@ struct DftCT {
 
 template <typename R> struct Dft<6, R> : DftCT<3, 2, R> {};
 template <typename R> struct Dft<8, R> : DftCT<4, 2, R> {};
+template <typename R> struct Dft<9, R> : DftCT<3, 3, R> {};
 template <typename R> struct Dft<10, R> : DftCT<5, 2, R> {};
 template <typename R> struct Dft<12, R> : DftCT<4, 3, R> {};
 template <typename R> struct Dft<16, R> : DftCT<4, 4, R> {};

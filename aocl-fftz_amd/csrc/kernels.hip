@@ -1,7 +1,8 @@
 // Launch table of the LDS pass kernels: precision x radix set x elements per
 // thread x block limit.  The kernels themselves are instantiated in
-// kernels_pass_{f32,f64}.hip and kernels_batch_{f32,f64}.hip (one translation
-// unit each, so they compile in parallel) from kernels_body.h.
+// kernels_pass_{f32,f64}_{mixed,pow2}.hip and
+// kernels_batch_{f32,f64}_{small,medium,large}.hip (one translation unit
+// each, so they compile in parallel) from kernels_body.h.
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
@@ -17,13 +18,16 @@ __global__ void k_reset_counters(unsigned* c, int n)
 
 const void* kernel_symbol(KernelKind kind, int prec_bytes, bool all_radix, int E, int maxt)
 {
-    return prec_bytes == 4 ? kernel_symbol_pass_f32(kind, all_radix, E, maxt)
-                           : kernel_symbol_pass_f64(kind, all_radix, E, maxt);
+    if (prec_bytes == 4)
+        return all_radix ? kernel_symbol_pass_f32_mixed(kind, E, maxt) : kernel_symbol_pass_f32_pow2(kind, E, maxt);
+    return all_radix ? kernel_symbol_pass_f64_mixed(kind, E, maxt) : kernel_symbol_pass_f64_pow2(kind, E, maxt);
 }
 
 const void* kernel_symbol_batch(int prec_bytes, int E, int maxt)
 {
-    return prec_bytes == 4 ? kernel_symbol_batch_f32(E, maxt) : kernel_symbol_batch_f64(E, maxt);
+    if (E == 8) return prec_bytes == 4 ? kernel_symbol_batch_f32_small() : kernel_symbol_batch_f64_small();
+    if (maxt == 512) return prec_bytes == 4 ? kernel_symbol_batch_f32_medium() : kernel_symbol_batch_f64_medium();
+    return prec_bytes == 4 ? kernel_symbol_batch_f32_large() : kernel_symbol_batch_f64_large();
 }
 
 const void* kernel_symbol_reset() { return (const void*)&k_reset_counters; }

@@ -1,0 +1,9 @@
+// Instantiation unit: uneven-batch persistent kernel, large class: 16 values per thread, 1024 threads, complex64.
+// (One kernel per unit: the units compile in parallel.)
+#include "kernels_body.h"
+
+namespace fftz {
+
+const void* kernel_symbol_batch_f32_large() { return (const void*)&k_batch<float, 16, 1024>; }
+
+}  // namespace fftz

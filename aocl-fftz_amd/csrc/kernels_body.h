@@ -15,9 +15,9 @@
 //   writes   y[g*R*ls + i*ls + c].
 // Reads and writes happen in place with a workgroup barrier in between; each
 // thread holds its K = ceil(E/R) butterflies in VGPRs across the barrier.
-// Included by the instantiation units kernels_pass_f32.hip, kernels_pass_f64.hip,
-// kernels_batch_f32.hip and kernels_batch_f64.hip (compiled in parallel);
-// kernels.hip holds the launch-table dispatch.
+// Included by the instantiation units kernels_pass_{f32,f64}_{mixed,pow2}.hip
+// and kernels_batch_{f32,f64}_{small,medium,large}.hip (compiled in
+// parallel); kernels.hip holds the launch-table dispatch.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -224,27 +224,82 @@ __device__ __forceinline__ void c2r_separate(const PassDesc& p, cplx<R>* lds, in
     }
 }
 
+// Legs b0 + i * stride (i < RAD) of a line in the padded LDS layout.  When
+// the stride is a multiple of 16 (LIN) the pad term carries nothing between
+// legs: lds_pad(b0 + i stride) = lds_pad(b0) + i (stride + stride / 16), so
+// each leg's slot is one add of a uniform step from the first.  Callers pick
+// LIN with a uniform branch so the two address forms are separate code
+// (complex64 only: the complex128 1024-thread variants, at their 128-VGPR
+// bound, spill with both forms present).
+template <typename R, int RAD, bool LIN>
+__device__ __forceinline__ void lds_gather(const cplx<R>* line, int b0, int stride, cplx<R> (&v)[RAD])
+{
+    if constexpr (LIN) {
+        const int step = stride + (stride >> 4);
+        const cplx<R>* q = line + lds_pad(b0);
+#pragma unroll
+        for (int i = 0; i < RAD; i++) v[i] = q[i * step];
+    } else {
+#pragma unroll
+        for (int i = 0; i < RAD; i++) v[i] = line[lds_pad(b0 + i * stride)];
+    }
+}
+
+template <typename R, int RAD, bool LIN>
+__device__ __forceinline__ void lds_scatter(cplx<R>* line, int b0, int stride, const cplx<R> (&v)[RAD])
+{
+    if constexpr (LIN) {
+        const int step = stride + (stride >> 4);
+        cplx<R>* q = line + lds_pad(b0);
+#pragma unroll
+        for (int i = 0; i < RAD; i++) q[i * step] = v[i];
+    } else {
+#pragma unroll
+        for (int i = 0; i < RAD; i++) line[lds_pad(b0 + i * stride)] = v[i];
+    }
+}
+
+// Stage twiddles held in registers across a stage's barrier: only while a
+// thread's share stays within 32 VGPRs (K (R - 1) <= 16 complex64 or 8
+// complex128 values), else they are fetched after it.
 template <typename R, int E, int RAD>
-__device__ __forceinline__ void stage(const PassDesc& p, cplx<R>* lds, int tid, int s)
+constexpr bool kTwEarly = ((E + RAD - 1) / RAD) * (RAD - 1) * (int)sizeof(R) <= 64;
+
+// One in-place Stockham stage out of LDS (LIN: both the read stride L / R and
+// the write stride ls are multiples of 16).
+template <typename R, int E, int RAD, bool LIN>
+__device__ __forceinline__ void stage_impl(const PassDesc& p, cplx<R>* lds, int tid, int s)
 {
     constexpr int K = (E + RAD - 1) / RAD;
+    constexpr bool EARLY = kTwEarly<R, E, RAD>;
     const int c = tid >> p.tl_shift;
     const int t = tid & (p.TL - 1);
     const int nbl = p.L / RAD;
     const int ls = p.ls[s];
     cplx<R>* line = lds + c * p.LP;
     const bool act = c < p.C;  // batch kernel: threads beyond C * TL idle
+    const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(p.stw) + p.twoff[s];
     cplx<R> v[K][RAD];
+    cplx<R> w[EARLY ? K : 1][RAD - 1];
+    if constexpr (EARLY) {
+        // fetched before the LDS reads: the latency overlaps them and the barrier
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int j = t + k * p.TL;
+            const int g = (int)fdiv((uint32_t)j, p.lsdiv[s]);
+            const int cc = j - g * ls;
+            if (act && j < nbl && ls > 1) {
+#pragma unroll
+                for (int i = 1; i < RAD; i++) w[k][i - 1] = tw[cc * (RAD - 1) + i - 1];
+            }
+        }
+    }
 #pragma unroll
     for (int k = 0; k < K; k++) {
         const int j = t + k * p.TL;
-        if (act && j < nbl) {
-#pragma unroll
-            for (int i = 0; i < RAD; i++) v[k][i] = line[lds_pad(i * nbl + j)];
-        }
+        if (act && j < nbl) lds_gather<R, RAD, LIN>(line, j, nbl, v[k]);
     }
     __syncthreads();
-    const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(p.stw) + p.twoff[s];
 #pragma unroll
     for (int k = 0; k < K; k++) {
         const int j = t + k * p.TL;
@@ -252,17 +307,28 @@ __device__ __forceinline__ void stage(const PassDesc& p, cplx<R>* lds, int tid, 
             const int g = (int)fdiv((uint32_t)j, p.lsdiv[s]);
             const int cc = j - g * ls;
             if (ls > 1) {
-                const cplx<R>* w = tw + cc * (RAD - 1);
 #pragma unroll
-                for (int i = 1; i < RAD; i++) v[k][i] = cmul(v[k][i], w[i - 1]);
+                for (int i = 1; i < RAD; i++) {
+                    if constexpr (EARLY)
+                        v[k][i] = cmul(v[k][i], w[k][i - 1]);
+                    else
+                        v[k][i] = cmul(v[k][i], tw[cc * (RAD - 1) + i - 1]);
+                }
             }
             Dft<RAD, R>::run(v[k]);
-            const int base = g * RAD * ls + cc;
-#pragma unroll
-            for (int i = 0; i < RAD; i++) line[lds_pad(base + i * ls)] = v[k][i];
+            lds_scatter<R, RAD, LIN>(line, g * RAD * ls + cc, ls, v[k]);
         }
     }
     __syncthreads();
+}
+
+template <typename R, int E, int RAD>
+__device__ __forceinline__ void stage(const PassDesc& p, cplx<R>* lds, int tid, int s)
+{
+    if (sizeof(R) == 4 && ((p.L / RAD) & 15) == 0 && (p.ls[s] & 15) == 0)
+        stage_impl<R, E, RAD, true>(p, lds, tid, s);
+    else
+        stage_impl<R, E, RAD, false>(p, lds, tid, s);
 }
 
 template <typename R, bool ALLRAD, int E>
@@ -281,6 +347,7 @@ __device__ __forceinline__ void run_stages(const PassDesc& p, cplx<R>* lds, int 
                         case 5: stage<R, E, 5>(p, lds, tid, s); break;
                         case 6: stage<R, E, 6>(p, lds, tid, s); break;
                         case 7: stage<R, E, 7>(p, lds, tid, s); break;
+                        case 9: stage<R, E, 9>(p, lds, tid, s); break;
                         case 10: stage<R, E, 10>(p, lds, tid, s); break;
                         case 11: stage<R, E, 11>(p, lds, tid, s); break;
                         case 12: stage<R, E, 12>(p, lds, tid, s); break;
@@ -568,9 +635,9 @@ __device__ __forceinline__ void store_butterfly(const PassDesc& p, cplx<R>* v, i
     }
 }
 
-template <typename R, int E, int RAD>
-__device__ __forceinline__ void last_stage_direct(const PassDesc& p, const cplx<R>* lds, int c, int t, cplx<R>* dst,
-                                                  uint32_t l0, bool live, int s, int64_t gbase)
+template <typename R, int E, int RAD, bool LIN>
+__device__ __forceinline__ void last_stage_direct_impl(const PassDesc& p, const cplx<R>* lds, int c, int t,
+                                                       cplx<R>* dst, uint32_t l0, bool live, int s, int64_t gbase)
 {
     constexpr int K = (E + RAD - 1) / RAD;
     const int nbl = p.L / RAD;  // = ls of the last stage; g = 0, cc = j
@@ -579,10 +646,7 @@ __device__ __forceinline__ void last_stage_direct(const PassDesc& p, const cplx<
 #pragma unroll
     for (int k = 0; k < K; k++) {
         const int j = t + k * p.TL;
-        if (live && j < nbl) {
-#pragma unroll
-            for (int i = 0; i < RAD; i++) v[k][i] = line[lds_pad(i * nbl + j)];
-        }
+        if (live && j < nbl) lds_gather<R, RAD, LIN>(line, j, nbl, v[k]);
     }
     const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(p.stw) + p.twoff[s];
 #pragma unroll
@@ -598,6 +662,16 @@ __device__ __forceinline__ void last_stage_direct(const PassDesc& p, const cplx<
             store_butterfly<R, RAD>(p, v[k], j, nbl, dst, l0, gbase);
         }
     }
+}
+
+template <typename R, int E, int RAD>
+__device__ __forceinline__ void last_stage_direct(const PassDesc& p, const cplx<R>* lds, int c, int t, cplx<R>* dst,
+                                                  uint32_t l0, bool live, int s, int64_t gbase)
+{
+    if (sizeof(R) == 4 && ((p.L / RAD) & 15) == 0)
+        last_stage_direct_impl<R, E, RAD, true>(p, lds, c, t, dst, l0, live, s, gbase);
+    else
+        last_stage_direct_impl<R, E, RAD, false>(p, lds, c, t, dst, l0, live, s, gbase);
 }
 
 // One-stage lengths (L <= 16): load, DFT, store, no LDS.
@@ -630,6 +704,7 @@ __device__ __forceinline__ void single_stage_direct(const PassDesc& p, int t, co
                     case 5: { constexpr int RAD = 5; CALL; } break;   \
                     case 6: { constexpr int RAD = 6; CALL; } break;   \
                     case 7: { constexpr int RAD = 7; CALL; } break;   \
+                    case 9: { constexpr int RAD = 9; CALL; } break;   \
                     case 10: { constexpr int RAD = 10; CALL; } break; \
                     case 11: { constexpr int RAD = 11; CALL; } break; \
                     case 12: { constexpr int RAD = 12; CALL; } break; \
@@ -839,7 +914,7 @@ __device__ __forceinline__ void bs_first(const PassDesc& p, cplx<R>* lds, int c,
     }
 }
 
-template <typename R, int E, int RAD>
+template <typename R, int E, int RAD, bool LIN>
 __device__ __forceinline__ void bs_last_common(const PassDesc& p, const cplx<R>* line, int t, bool live, int s,
                                                cplx<R> (&v)[(E + RAD - 1) / RAD][RAD])
 {
@@ -848,10 +923,7 @@ __device__ __forceinline__ void bs_last_common(const PassDesc& p, const cplx<R>*
 #pragma unroll
     for (int k = 0; k < K; k++) {
         const int j = t + k * p.TL;
-        if (live && j < nbl) {
-#pragma unroll
-            for (int i = 0; i < RAD; i++) v[k][i] = line[lds_pad(i * nbl + j)];
-        }
+        if (live && j < nbl) lds_gather<R, RAD, LIN>(line, j, nbl, v[k]);
     }
     const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(p.stw) + p.twoff[s];
 #pragma unroll
@@ -868,38 +940,45 @@ __device__ __forceinline__ void bs_last_common(const PassDesc& p, const cplx<R>*
     }
 }
 
-template <typename R, int E, int RAD>
-__device__ __forceinline__ void bs_spec(const PassDesc& p, cplx<R>* lds, int c, int t, bool live, int s)
+template <typename R, int E, int RAD, bool LIN>
+__device__ __forceinline__ void bs_spec_impl(const PassDesc& p, cplx<R>* lds, int c, int t, bool live, int s)
 {
     constexpr int K = (E + RAD - 1) / RAD;
     const int nbl = p.L / RAD;
     cplx<R>* line = lds + c * p.LP;
     cplx<R> v[K][RAD];
-    bs_last_common<R, E, RAD>(p, line, t, live, s, v);
+    bs_last_common<R, E, RAD, LIN>(p, line, t, live, s, v);
     const cplx<R>* spec = reinterpret_cast<const cplx<R>*>(p.spec);
 #pragma unroll
     for (int k = 0; k < K; k++) {
         const int j = t + k * p.TL;
         if (live && j < nbl) {
 #pragma unroll
-            for (int i = 0; i < RAD; i++) {
-                const int e = i * nbl + j;
-                line[lds_pad(e)] = cmul(cconj(v[k][i]), spec[e]);
-            }
+            for (int i = 0; i < RAD; i++) v[k][i] = cmul(cconj(v[k][i]), spec[i * nbl + j]);
+            lds_scatter<R, RAD, LIN>(line, j, nbl, v[k]);
         }
     }
 }
 
 template <typename R, int E, int RAD>
-__device__ __forceinline__ void bs_final(const PassDesc& p, const cplx<R>* lds, int c, int t, cplx<R>* dst, bool live,
-                                         int s)
+__device__ __forceinline__ void bs_spec(const PassDesc& p, cplx<R>* lds, int c, int t, bool live, int s)
+{
+    if (sizeof(R) == 4 && ((p.L / RAD) & 15) == 0)
+        bs_spec_impl<R, E, RAD, true>(p, lds, c, t, live, s);
+    else
+        bs_spec_impl<R, E, RAD, false>(p, lds, c, t, live, s);
+}
+
+template <typename R, int E, int RAD, bool LIN>
+__device__ __forceinline__ void bs_final_impl(const PassDesc& p, const cplx<R>* lds, int c, int t, cplx<R>* dst,
+                                              bool live, int s)
 {
     constexpr int K = (E + RAD - 1) / RAD;
     const int nbl = p.L / RAD;
     const int n = p.nvalid_out;
     const cplx<R>* line = lds + c * p.LP;
     cplx<R> v[K][RAD];
-    bs_last_common<R, E, RAD>(p, line, t, live, s, v);
+    bs_last_common<R, E, RAD, LIN>(p, line, t, live, s, v);
     const cplx<R>* chirp = reinterpret_cast<const cplx<R>*>(p.chirp);
     const bool fwd = !(p.flags & kConjOut);
 #pragma unroll
@@ -920,6 +999,16 @@ __device__ __forceinline__ void bs_final(const PassDesc& p, const cplx<R>* lds, 
             }
         }
     }
+}
+
+template <typename R, int E, int RAD>
+__device__ __forceinline__ void bs_final(const PassDesc& p, const cplx<R>* lds, int c, int t, cplx<R>* dst, bool live,
+                                         int s)
+{
+    if (sizeof(R) == 4 && ((p.L / RAD) & 15) == 0)
+        bs_final_impl<R, E, RAD, true>(p, lds, c, t, dst, live, s);
+    else
+        bs_final_impl<R, E, RAD, false>(p, lds, c, t, dst, live, s);
 }
 
 template <typename R, bool ALLRAD, int E>

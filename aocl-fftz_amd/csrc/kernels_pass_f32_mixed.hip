@@ -1,0 +1,12 @@
+// Instantiation unit: single-size pass and Bluestein kernels with every radix (2-13),
+// complex64; 8 or 16 values per thread, 256- or 1024-thread bound.
+#include "kernels_body.h"
+
+namespace fftz {
+
+const void* kernel_symbol_pass_f32_mixed(KernelKind kind, int E, int maxt)
+{
+    return sym_et<float, true>(kind, E, maxt);
+}
+
+}  // namespace fftz

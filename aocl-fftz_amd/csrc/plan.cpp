@@ -160,7 +160,7 @@ static void host_fft(int64_t n, const d2* in, d2* out)
 // ---------------------------------------------------------------------------
 // Factorisation helpers
 // ---------------------------------------------------------------------------
-static const int kRadices[] = {16, 13, 12, 11, 10, 8, 7, 6, 5, 4, 3, 2};
+static const int kRadices[] = {16, 13, 12, 11, 10, 9, 8, 7, 6, 5, 4, 3, 2};
 
 static bool smooth13(int64_t n)
 {

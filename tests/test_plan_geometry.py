@@ -38,6 +38,16 @@ def test_1d_batched(n, prec):
     c2c_case([(n, 1, 1)], [(b, n, n)], prec)
 
 
+@pytest.mark.parametrize("n,radix", [(6561, [9, 9, 9, 9]), (2187, [9, 9, 9, 3]), (3003, [13, 11, 7, 3]),
+                                     (1008, [16, 9, 7]), (4096, [16, 16, 16])])
+def test_default_radix_plan(n, radix):
+    """Fewest stages from {16, 13, 12, 11, 10, 9, 8, 7, 6, 5, 4, 3, 2}, largest
+    radix first: powers of three run as radix-9 stages (3^8 in four stages,
+    not eight)."""
+    plan = c2c_case([(n, 1, 1)], [(2, n, n)], "f")
+    assert plan["launches"][0]["radix"] == radix
+
+
 @pytest.mark.parametrize("n,prec,direction", [(16411, "f", 0), (16411, "f", 1), (8191, "d", 0), (4099, "d", 1),
                                               (65537, "d", 0)])
 def test_bluestein_multi_pass(n, prec, direction):
