@@ -1054,6 +1054,18 @@ __global__ void __launch_bounds__(MAXT) k_bluestein(const PassDesc p_arg)
         bluestein_body<R, ALLRAD, E>(p, io, lds, (int64_t)blockIdx.x * p.C, threadIdx.x);
 }
 
+// Length-1 items of an uneven batch: the transform is the identity (the
+// backward direction's two conjugations cancel).
+template <typename R>
+__device__ __forceinline__ void batch_identity(const PassDesc& p, const TileIo& io, int64_t l_base, int tid)
+{
+    const int c = tid >> p.tl_shift;
+    const SlotLine sl = slot_line(p, io, l_base, c);
+    if (c >= p.C || !sl.live || (tid & (p.TL - 1)) != 0) return;
+    const LineIdx li = line_index(p, io, sl.l);
+    reinterpret_cast<cplx<R>*>(io.out)[li.ooff] = reinterpret_cast<const cplx<R>*>(io.in)[li.ioff];
+}
+
 // ---------------------------------------------------------------------------
 // Uneven batches in one persistent launch.  A device work queue lists tiles
 // (size class, first line, line count, offsets); each workgroup takes the
@@ -1093,14 +1105,15 @@ k_batch(const BatchArgs a)
         const int C = p.C;
         // a queue entry covers one or more rounds of C lines
         for (int64_t lb = 0; lb < io.nlines; lb += C) {
-            if (blue) {
-                if (p.nst >= 2)
-                    bluestein_body_direct<R, true, E>(p, io, lds, lb, tid);
-                else
-                    bluestein_body<R, true, E>(p, io, lds, lb, tid);
-            } else {
-                pass_body<R, true, E>(p, io, lds, lb, tid);
-            }
+            // only the register-direct bodies: batch items are plain complex
+            // transforms (no staged real-packed steps), and a Bluestein
+            // extended length (M >= 33) always has two or more stages
+            if (blue)
+                bluestein_body_direct<R, true, E>(p, io, lds, lb, tid);
+            else if (p.nst >= 1)
+                pass_body_direct<R, true, E>(p, io, lds, lb, tid);
+            else
+                batch_identity<R>(p, io, lb, tid);
             __syncthreads();
         }
         if (tid == 0) s_tile = nxt;

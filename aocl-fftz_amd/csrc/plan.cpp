@@ -1313,6 +1313,11 @@ static bool batch_size_desc(TableBuilder& tb, int prec, int64_t n, int E, int ma
     }
     d.L = (int)L;
     d.nst = (int)radix.size();
+    if (si.bluestein && d.nst < 2) {
+        // the batch kernel runs the register-direct Bluestein body only
+        err = "internal: Bluestein batch length " + std::to_string(L) + " with one stage";
+        return false;
+    }
     for (size_t s = 0; s < radix.size(); s++) d.radix[s] = radix[s];
     const int64_t stw = stage_twiddles(tb, prec, radix, d.twoff, d.ls);
     for (int s = 0; s < d.nst; s++) d.lsdiv[s] = make_fastdiv((uint32_t)d.ls[s]);
