@@ -27,6 +27,7 @@
 #include <cstring>
 #include <functional>
 #include <map>
+#include <mutex>
 #include <sstream>
 #include <thread>
 #include <atomic>
@@ -708,6 +709,82 @@ static int64_t bluestein_length(int64_t n)
         if (smooth13(m)) return m;
 }
 
+// Estimated vector instructions per element for one generic LDS stage of
+// radix r: the in-register DFT-r (real operations per element of the codelet
+// in codelets.h), the stage-twiddle product ((r - 1) / r complex products)
+// and the LDS addressing of one read and one write.
+static double radix_stage_cost(int r)
+{
+    switch (r) {
+        case 2: return 17.5;
+        case 3: return 19.5;
+        case 4: return 19.5;
+        case 5: return 22.0;
+        case 6: return 22.5;
+        case 7: return 25.0;
+        case 8: return 22.0;
+        case 9: return 26.0;
+        case 10: return 25.5;
+        case 11: return 29.0;
+        case 12: return 26.5;
+        case 13: return 31.0;
+        default: return 25.0;  // 16
+    }
+}
+
+// Extended length of a Bluestein transform that runs in one LDS tile.  Any
+// 13-smooth M >= 2n - 1 gives the same transform; the reference takes the
+// smallest (bluestein_length above, which the multi-pass form keeps).  In one
+// tile the work is two M-point transforms through the LDS stages of M's
+// fewest-stage radix plan, so this takes the M with the least
+// M x sum(radix_stage_cost) among the 13-smooth lengths from the smallest up
+// to the next power of two (the same thread count and tile class) and the
+// tile limit.  E.g. n = 4099: 8232 = 12 7 7 7 2 becomes 8448 = 16 16 11 3.
+// Lengths beyond the limit are returned as the smallest (the caller plans
+// them multi-pass).
+static double transform_stage_cost(int64_t m)
+{
+    std::vector<int> r;
+    if (!smooth13(m) || !factor_radices((int)m, r)) return -1.0;
+    double per = 0.0;
+    for (int v : r) per += radix_stage_cost(v);
+    return (double)m * per;
+}
+
+// transform_stage_cost of every length up to 16384 (-1 where there is no
+// radix plan), built once per process
+static const std::vector<double>& stage_cost_table()
+{
+    static std::vector<double> t;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        t.assign(16385, -1.0);
+        for (int64_t m = 2; m <= 16384; m++) t[(size_t)m] = transform_stage_cost(m);
+    });
+    return t;
+}
+
+static int64_t bluestein_tile_length(int64_t n, int64_t limit)
+{
+    const int64_t m0 = bluestein_length(n);
+    if (m0 > limit || !env_int("FFTZ_BLUESTEIN_COST", 1)) return m0;
+    int64_t hi = 1;
+    while (hi < m0) hi <<= 1;
+    hi = std::min(hi, limit);
+    const std::vector<double>& tab = stage_cost_table();
+    int64_t best = m0;
+    double best_cost = 0.0;
+    for (int64_t m = m0; m <= hi; m++) {
+        const double cost = m < (int64_t)tab.size() ? tab[(size_t)m] : transform_stage_cost(m);
+        if (cost < 0.0) continue;
+        if (m == m0 || cost < best_cost) {
+            best = m;
+            best_cost = cost;
+        }
+    }
+    return best;
+}
+
 Plan* plan_create(const ProblemSpec& spec, std::string& err, bool dry_run);
 hipError_t plan_enqueue(Plan* p, Slot* s, const void* in, void* out, hipStream_t stream);
 
@@ -1120,7 +1197,7 @@ static bool plan_axis_c2c(Plan* plan, TableBuilder& tb, const std::vector<AxisSp
         return build_pass(plan, tb, q, err);
     }
     // Bluestein
-    const int64_t M = bluestein_length(n);
+    const int64_t M = bluestein_tile_length(n, single_pass_limit(prec));
     if (M > single_pass_limit(prec))
         return plan_bluestein_long(plan, tb, n, q.ies, q.oes, batch, in_role, out_role, conj_in, err);
     q.kind = KernelKind::Bluestein;
@@ -1165,7 +1242,7 @@ static bool real_full_pass(Plan* plan, TableBuilder& tb, PassReq q, int64_t n, s
         q.L = (int)n;
         return build_pass(plan, tb, q, err);
     }
-    const int64_t M = bluestein_length(n);
+    const int64_t M = bluestein_tile_length(n, single_pass_limit(prec));
     if (M > single_pass_limit(prec)) {
         err = "Bluestein length " + std::to_string(M) + " for real length " + std::to_string(n) +
               " exceeds the fused single-pass limit";
@@ -1302,7 +1379,7 @@ static bool batch_size_desc(TableBuilder& tb, int prec, int64_t n, int E, int ma
 {
     si.n = n;
     si.bluestein = !smooth13(n);
-    const int64_t L = si.bluestein ? bluestein_length(n) : n;
+    const int64_t L = si.bluestein ? bluestein_tile_length(n, single_pass_limit(prec)) : n;
     si.L = (int)L;
     PassDesc& d = si.d;
     d = PassDesc{};
@@ -1405,7 +1482,7 @@ Plan* batch_plan_create(const std::vector<BatchItem>& items, int prec, int dir, 
         if (size_index.count(it.n)) continue;
         SizeInfo si;
         si.n = it.n;
-        const int64_t L = smooth13(it.n) ? it.n : bluestein_length(it.n);
+        const int64_t L = smooth13(it.n) ? it.n : bluestein_tile_length(it.n, single_pass_limit(prec));
         const bool small = next_pow2((L + 7) / 8) <= 256 && (size_t)(lds_pad((int)L) + 2) * esize <= small_budget / 2;
         const bool medium = !small && next_pow2((L + 15) / 16) <= 512 &&
                             (size_t)(lds_pad((int)L) + 2) * esize <= medium_budget;

@@ -48,6 +48,26 @@ def test_default_radix_plan(n, radix):
     assert plan["launches"][0]["radix"] == radix
 
 
+@pytest.mark.parametrize("n,prec,M,radix", [(4099, "f", 8448, [16, 16, 11, 3]), (5099, "f", 10240, [16, 16, 10, 4]),
+                                            (101, "f", 208, [16, 13]), (1009, "d", 2048, [16, 16, 8])])
+def test_bluestein_tile_length_by_stage_cost(n, prec, M, radix):
+    """A Bluestein transform in one LDS tile takes, among the 13-smooth
+    extended lengths from the smallest (2n - 1 and up, the reference's rule)
+    to the next power of two, the one with the least estimated stage work;
+    the plan replays to the DFT."""
+    plan = c2c_case([(n, 1, 1)], [(2, n, n)], prec)
+    ln = plan["launches"][0]
+    assert ln["kind"] == "bluestein" and ln["L"] == M and ln["radix"] == radix
+
+
+def test_bluestein_multi_pass_keeps_smallest_length():
+    """Beyond one tile (complex128 M > 8192) the extended length stays the
+    reference's smallest 13-smooth M >= 2n - 1 (4099 -> 8232)."""
+    plan = c2c_case([(4099, 1, 1)], [(1, 4099, 4099)], "d")
+    assert all(ln["kind"] == "pass" for ln in plan["launches"])
+    assert plan["work_bytes"] == 2 * 8232 * 16
+
+
 @pytest.mark.parametrize("n,prec,direction", [(16411, "f", 0), (16411, "f", 1), (8191, "d", 0), (4099, "d", 1),
                                               (65537, "d", 0)])
 def test_bluestein_multi_pass(n, prec, direction):
