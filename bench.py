@@ -729,8 +729,8 @@ def _ref_bench_time(args, timeout=240):
 
 def cpu_baseline():
     """The reference's OpenMP CPU path on this host's cores, through its own
-    benchmark program built from its sources (BASELINE.md section 2 steps
-    2-3): the headline 2^24 complex128 in-place transform and the batched
+    benchmark program built from its sources (min of calibrated rounds,
+    test/bench/performance.c:36-125): the 2^24 complex128 in-place transform and the batched
     N=1024 x 4096 complex64 transform, each at 1 thread and at the host's
     thread count(s).  Runs before this process touches the GPU."""
     exe = os.path.join(ROOT, "oracle", "_ref", "aocl_fftz_bench")
