@@ -45,6 +45,26 @@ def test_batch_queue_emulated(prec, direction):
         assert O.rel_l2(y[oo:oo + n * b], ref) < 1e-12
 
 
+@pytest.mark.parametrize("prec", ["f", "d"])
+def test_batch_queue_edge_lengths(prec):
+    """Length-1 items, one-stage lengths and small Bluestein primes in one
+    queue: every transform covered once, in bounds, equal to the DFT."""
+    sizes = [(1, 5), (2, 3), (3, 1), (5, 4), (7, 2), (8, 6), (13, 3), (16, 2), (17, 4), (31, 1), (97, 3),
+             (1, 1), (101, 2), (1024, 2), (4099, 1), (2, 64), (15, 9)]
+    items, off = [], 0
+    for n, b in sizes:
+        items.append((n, b, off, off))
+        off += n * b
+    plan = fftz.batch_plan_json(items, prec, 0)
+    rng = np.random.default_rng(7)
+    x = rng.uniform(-1, 1, off) + 1j * rng.uniform(-1, 1, off)
+    y, seen = emu.run_batch(plan, x, 0)
+    assert len(seen) == sum(b for _, b in sizes)
+    for n, b, io, oo in items:
+        ref = O.fft(x[io:io + n * b].reshape(b, n), -1).reshape(-1)
+        assert O.rel_l2(y[oo:oo + n * b], ref) < 1e-12
+
+
 def test_full_size_queue_shape():
     items, total = workloads.uneven_batch(20000, seed=3)
     plan = fftz.batch_plan_json(items, "f", 0)

@@ -59,6 +59,34 @@ def test_small_uneven_all_items(torch_gpu, prec, direction):
     p.destroy()
 
 
+EDGE_ITEMS = [(1, 5), (2, 3), (3, 1), (5, 4), (7, 2), (8, 6), (13, 3), (16, 2), (17, 4), (31, 1), (97, 3),
+              (1, 1), (101, 2), (1024, 2), (4099, 1), (2, 64), (15, 9)]
+
+
+@pytest.mark.parametrize("prec", ["f", "d"])
+@pytest.mark.parametrize("direction", [0, 1])
+def test_uneven_edge_lengths(torch_gpu, prec, direction):
+    """Length-1 items (copied), one-stage lengths (2..16, register-only),
+    small primes whose Bluestein length is the smallest two-stage one
+    (17 -> 33), and long ones, mixed in one queue; every item against the
+    oracle."""
+    torch = torch_gpu
+    items, total = pack(EDGE_ITEMS)
+    dt = torch.complex64 if prec == "f" else torch.complex128
+    x = rand_dev(torch, total, 9, dt)
+    y = torch.full_like(x, float("nan"))
+    p = fftz.BatchPlan(items, prec, direction)
+    assert p.execute_io(x, y) == 0
+    torch.cuda.synchronize()
+    xh, yh = x.cpu().numpy(), y.cpu().numpy()
+    sign = -1 if direction == 0 else 1
+    for n, b, io, oo in items:
+        ref = O.fft(xh[io:io + n * b].reshape(b, n).astype(np.complex128), sign)
+        err = O.rel_l2(yh[oo:oo + n * b], ref.reshape(-1))
+        assert err <= tol(max(n, 2), prec), (n, b, err, p.info)
+    p.destroy()
+
+
 def test_uneven_full_size(torch_gpu):
     """20,000 items, lengths from seed 3 (70% 7-smooth in [16, 16384], 30%
     primes in [101, 8191]), batch 1..64, complex64, packed contiguously."""
