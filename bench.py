@@ -475,11 +475,44 @@ def bench_r2c_512(torch, fftz, steps, warmup, n=512):
     return res
 
 
-def bench_uneven(torch, fftz, steps, warmup):
+def uneven_parts(torch, fftz, items, x, y, steps):
+    """The uneven workload's 7-smooth items and prime (Bluestein) items, each
+    planned, autotuned and timed alone on the same buffers: where the batch's
+    time goes."""
+    import workloads
+    smooth = set(workloads.smooth7())
+    out = {}
+    for name, sel in (("smooth", [it for it in items if it[0] in smooth]),
+                      ("bluestein", [it for it in items if it[0] not in smooth])):
+        p = fftz.BatchPlan(sel, "f", 0)
+        assert p.autotune(x, y, reps=2) == 0
+        assert p.execute_io(x, y) == 0
+        torch.cuda.synchronize()
+        t0 = torch.cuda.Event(enable_timing=True)
+        t1 = torch.cuda.Event(enable_timing=True)
+        t0.record()
+        for _ in range(steps):
+            assert p.execute_io(x, y) == 0
+        t1.record()
+        torch.cuda.synchronize()
+        p.profile(True)
+        assert p.execute_io(x, y) == 0
+        torch.cuda.synchronize()
+        prof = p.profile_read()
+        p.profile(False)
+        out[name] = {"items": len(sel), "elements": int(sum(n * b for n, b, *_ in sel)),
+                     "ms_per_execute": round(t0.elapsed_time(t1) / steps, 4),
+                     "class_ms": [round(v, 3) for v in prof[-1]] if prof else None}
+        p.destroy()
+    return out
+
+
+def bench_uneven(torch, fftz, steps, warmup, parts=False):
     """Uneven batch: 20,000 complex64 C2C items from seed 3 (70% 7-smooth
     lengths in [16, 16384], 30% primes in [101, 8191] on the Bluestein path),
     batch 1..64 each, packed contiguously, forward, out of place; one
-    persistent launch per tile class, replayed as a graph."""
+    persistent launch per tile class, replayed as a graph.  parts: also the
+    smooth and prime items timed alone (uneven_parts)."""
     import workloads
     items, total = workloads.uneven_batch()
     g = torch.Generator(device="cuda")
@@ -530,6 +563,8 @@ def bench_uneven(torch, fftz, steps, warmup):
         "plan": p.info,
     }
     p.destroy()
+    if parts:
+        res["parts"] = uneven_parts(torch, fftz, items, x, y, steps)
     del x, y
     torch.cuda.empty_cache()
     return res
@@ -792,7 +827,8 @@ def main():
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM-traffic runs")
     ap.add_argument("--traffic-probe", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--legs", default="mixed,r2c,uneven,split,serving",
-                    help="extra workloads after the primary and N=1024 legs (comma list; '' for none)")
+                    help="extra workloads after the primary and N=1024 legs (comma list; '' for none; "
+                         "uneven_parts: the uneven batch plus its smooth and prime items timed alone)")
     args = ap.parse_args()
     if args.traffic_probe:
         traffic_probe(args.traffic_probe)
@@ -836,10 +872,12 @@ def main():
         log("[bench] start 512^3 R2C fp32")
         rr = bench_r2c_512(torch, fftz, args.steps, args.warmup)
         log(f"[bench] r2c 512^3: {rr['ms_per_execute']:.3f} ms {rr['gflops']:.1f} GFLOP/s err {rr['rel_l2_vs_fp64']:.2e}")
-    if "uneven" in legs:
+    if "uneven" in legs or "uneven_parts" in legs:
         log("[bench] start uneven batch (20,000 items) fp32")
-        ru = bench_uneven(torch, fftz, args.steps, args.warmup)
+        ru = bench_uneven(torch, fftz, args.steps, args.warmup, parts="uneven_parts" in legs)
         log(f"[bench] uneven: {ru['ms_per_execute']:.3f} ms {ru['gflops']:.1f} GFLOP/s frac {ru['frac_of_hbm']:.3f}")
+        for name, r in ru.get("parts", {}).items():
+            log(f"[bench] uneven {name} items alone: {r['ms_per_execute']:.3f} ms classes {r['class_ms']}")
     if "split" in legs:
         log("[bench] start split-precision matrix-core vs plain, N=4096 batch 16384")
         rs = bench_split(torch, fftz, args.steps, args.warmup)
